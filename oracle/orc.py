@@ -1,0 +1,287 @@
+"""TEST INFRASTRUCTURE ONLY: Python access to the FP64 oracle.
+
+Two checkers live here:
+
+* ``C``   -- the C restatement ``oracle/mlcp_oracle.c`` (built by
+  ``oracle/Makefile`` into ``oracle/libmlcp_oracle.so``);
+* ``Ref`` -- the compiled reference Fortran routines driven by
+  ``oracle/ref/ref_driver.f90`` (built into ``oracle/_ref/ref_driver`` from the
+  reference sources; absent where the sources are absent).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
+this module; the HIP library never does.
+"""
+from __future__ import annotations
+
+import ctypes as C_
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+from mlcp_amd.hoplist import CPBlock, HopList
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmlcp_oracle.so")
+REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i, _d = C_.c_int, C_.c_double
+_H = [_i, _ip, _i, _ip, _ip, _dp, _dp]        # ndof,nbas,T,opid,mptr,hcoef,mat
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {LIB_PATH}")
+        L = C_.CDLL(LIB_PATH)
+        L.orc_dspmv_u.argtypes = [_i, _d, _dp, _dp, _dp]
+        L.orc_dspmv_u.restype = None
+        L.orc_prodvv.argtypes = [_i, _ip, _i, _dp, _dp, _i, _dp, _dp]
+        L.orc_prodvv.restype = _d
+        L.orc_buildbjk.argtypes = _H + [_i, _dp, _dp, _dp, _dp, _i, _i, _d,
+                                        _i, _dp, _dp]
+        L.orc_als_pow3.argtypes = _H + [_i, _dp, _dp, _i, _i, _d]
+        L.orc_prodhv_als3.argtypes = _H + [_i, _dp, _dp, _dp, _dp, _i, _i, _d]
+        L.orc_normbase.argtypes = [_i, _ip, _i, _dp, _dp]
+        L.orc_normbase.restype = None
+        L.orc_normcoef.argtypes = [_i, _ip, _i, _dp, _dp]
+        L.orc_normcoef.restype = None
+        L.orc_als_ortho3.argtypes = [_i, _ip, _i, _ip, _dp, _dp, _i]
+        L.orc_getqhq3.argtypes = _H + [_i, _ip, _dp, _dp, _i, _dp]
+        L.orc_getoverlaps.argtypes = [_i, _ip, _i, _ip, _dp, _dp, _dp]
+        L.orc_getoverlaps.restype = None
+        L.orc_dsygv.argtypes = [_i, _dp, _dp, _dp]
+        L.orc_als_sumlcvec3.argtypes = [_i, _ip, _i, _dp, _dp, _i, _ip, _dp,
+                                        _dp, _dp, _i]
+        L.orc_intertwine.argtypes = [_i, _ip, _i, _ip, _dp, _dp, _i, _ip, _ip,
+                                     _dp, _dp, _dp, _i, _i, _i, _i, _d]
+        L.orc_reduc_als.argtypes = [_i, _ip, _i, _dp, _dp, _i, _dp, _dp, _i]
+        L.orc_rayleigh.argtypes = _H + [_i, _ip, _dp, _dp, _dp]
+        L.orc_rayleigh.restype = None
+        _lib = L
+    return _lib
+
+
+def _h(H: HopList):
+    return (H.ndof, H.nbas, H.T, H.opid, H.mptr, H.coef, H.mat)
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with code {rc}")
+
+
+class C:
+    """The C restatement.  Block arguments are copied; results returned."""
+
+    @staticmethod
+    def dspmv(ap, x, fac=1.0):
+        y = np.zeros_like(x)
+        lib().orc_dspmv_u(x.size, fac, np.ascontiguousarray(ap),
+                          np.ascontiguousarray(x), y)
+        return y
+
+    @staticmethod
+    def prodvv(nbas, fcoef, fbase, gcoef=None, gbase=None):
+        nbas = np.ascontiguousarray(nbas, dtype=np.int32)
+        if gcoef is None:
+            return lib().orc_prodvv(nbas.size, nbas, fcoef.size, fcoef, fbase,
+                                    fcoef.size, fcoef, fbase)
+        return lib().orc_prodvv(nbas.size, nbas, fcoef.size, fcoef, fbase,
+                                gcoef.size, gcoef, gbase)
+
+    @staticmethod
+    def buildbjk(H, ocoef, obase, coef, base, k, ishift, eshift, recalc):
+        R = coef.size
+        BB = np.zeros((R, R))            # column major via transpose at end
+        bjk = np.zeros((int(H.nbas[k]), R))
+        _chk(lib().orc_buildbjk(*_h(H), R, ocoef, obase, coef, base, k,
+                                ishift, eshift, int(recalc), BB, bjk),
+             "buildbjk")
+        return BB.T.copy(), bjk.T.copy()   # Fortran BB(i,l), bjk(l,j)
+
+    @staticmethod
+    def als_pow3(H, coef, base, nitn, ishift=1, eshift=0.0):
+        c, b = coef.copy(), base.copy()
+        _chk(lib().orc_als_pow3(*_h(H), c.size, c, b, nitn, ishift, eshift),
+             "als_pow3")
+        return c, b
+
+    @staticmethod
+    def prodhv_als3(H, coef, base, nitn, ishift=0, eshift=0.0):
+        c, b = np.empty_like(coef), np.empty_like(base)
+        _chk(lib().orc_prodhv_als3(*_h(H), coef.size, coef, base, c, b, nitn,
+                                   ishift, eshift), "prodhv_als3")
+        return c, b
+
+    @staticmethod
+    def normcoef(nbas, coef, base):
+        c, b = coef.copy(), base.copy()
+        lib().orc_normcoef(nbas.size, nbas, c.size, c, b)
+        return c, b
+
+    @staticmethod
+    def als_ortho3(Q: CPBlock, nitn) -> CPBlock:
+        out = Q.copy()
+        _chk(lib().orc_als_ortho3(out.nbas.size, out.nbas, out.nbloc,
+                                  out.ranks, out.coef, out.base, nitn),
+             "als_ortho3")
+        return out
+
+    @staticmethod
+    def getqhq3(H, Q: CPBlock, nitn):
+        n = Q.nbloc
+        M = np.zeros((n, n))
+        _chk(lib().orc_getqhq3(*_h(H), n, Q.ranks, Q.coef, Q.base, nitn, M),
+             "getqhq3")
+        return M.T.copy()
+
+    @staticmethod
+    def getoverlaps(Q: CPBlock):
+        n = Q.nbloc
+        S = np.zeros((n, n))
+        lib().orc_getoverlaps(Q.nbas.size, Q.nbas, n, Q.ranks, Q.coef, Q.base,
+                              S)
+        return S.T.copy()
+
+    @staticmethod
+    def dsygv(A, B):
+        """A, B: (n, n) with upper triangles set.  Returns (w, X)."""
+        n = A.shape[0]
+        a = np.ascontiguousarray(A.T)
+        b = np.ascontiguousarray(B.T)
+        w = np.zeros(n)
+        _chk(lib().orc_dsygv(n, a, b, w), "dsygv")
+        return w, a.T.copy()
+
+    @staticmethod
+    def sumlcvec3(F_coef, F_base, Q: CPBlock, lc, nitn):
+        c, b = F_coef.copy(), F_base.copy()
+        _chk(lib().orc_als_sumlcvec3(Q.nbas.size, Q.nbas, c.size, c, b,
+                                     Q.nbloc, Q.ranks, Q.coef, Q.base,
+                                     np.ascontiguousarray(lc, dtype=np.float64),
+                                     nitn), "sumlcvec3")
+        return c, b
+
+    @staticmethod
+    def intertwine(H, Q: CPBlock, npow, nals, update=True, ishift=1,
+                   eshift=0.0) -> CPBlock:
+        out = Q.copy()
+        _chk(lib().orc_intertwine(out.nbas.size, out.nbas, out.nbloc,
+                                  out.ranks, out.coef, out.base, H.T, H.opid,
+                                  H.mptr, H.coef, H.mat, out.eigv, npow, nals,
+                                  int(update), ishift, eshift), "intertwine")
+        return out
+
+    @staticmethod
+    def reduc_als(nbas, gcoef, gbase, fcoef, fbase, nitn):
+        c, b = fcoef.copy(), fbase.copy()
+        _chk(lib().orc_reduc_als(nbas.size, nbas, gcoef.size, gcoef, gbase,
+                                 c.size, c, b, nitn), "reduc_als")
+        return c, b
+
+    @staticmethod
+    def rayleigh(H, Q: CPBlock):
+        rq = np.zeros(Q.nbloc)
+        lib().orc_rayleigh(*_h(H), Q.nbloc, Q.ranks, Q.coef, Q.base, rq)
+        return rq
+
+
+class Ref:
+    """The compiled reference routines, through oracle/_ref/ref_driver."""
+
+    MAGIC = 1296843600
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(REF_DRIVER)
+
+    @staticmethod
+    def run(mode, H: HopList, Q: CPBlock, npow=0, nals=0, update=0, ishift=1,
+            eshift=0.0, extra=None):
+        extra = np.zeros(0) if extra is None else np.ascontiguousarray(
+            extra, dtype=np.float64).ravel(order="F")
+        hdr = np.array([Ref.MAGIC, mode, H.ndof, Q.nbloc, H.T, H.nops, H.msize,
+                        npow, nals, update, ishift, extra.size], dtype=np.int32)
+        with tempfile.TemporaryDirectory() as td:
+            fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+            with open(fin, "wb") as f:
+                f.write(hdr.tobytes())
+                f.write(np.float64(eshift).tobytes())
+                for a in (H.nbas, Q.ranks, H.opid, H.mptr):
+                    f.write(np.ascontiguousarray(a, dtype=np.int32).tobytes())
+                for a in (H.coef, H.mat, Q.coef, Q.base, Q.eigv, extra):
+                    f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+            p = subprocess.run([REF_DRIVER, fin, fout], capture_output=True,
+                               text=True, timeout=600)
+            if p.returncode != 0 or not os.path.exists(fout):
+                raise RuntimeError(f"ref_driver failed: {p.stdout}{p.stderr}")
+            raw = open(fout, "rb").read()
+        status = np.frombuffer(raw[:4], dtype=np.int32)[0]
+        if status != 0:
+            raise RuntimeError(f"ref_driver status {status}")
+        return np.frombuffer(raw[4:], dtype=np.float64).copy()
+
+    @staticmethod
+    def _block(vals, Q: CPBlock):
+        nt, nD = Q.base.shape
+        out = Q.copy()
+        out.coef = vals[:nt].copy()
+        out.base = vals[nt:nt + nt * nD].reshape(nt, nD).copy()
+        return out, vals[nt + nt * nD:]
+
+    @staticmethod
+    def als_pow3(H, Q, npow, ishift=1, eshift=0.0):
+        return Ref._block(Ref.run(1, H, Q, npow=npow, ishift=ishift,
+                                  eshift=eshift), Q)[0]
+
+    @staticmethod
+    def prodhv_als3(H, Q, nals, ishift=0, eshift=0.0):
+        return Ref._block(Ref.run(2, H, Q, nals=nals, ishift=ishift,
+                                  eshift=eshift), Q)[0]
+
+    @staticmethod
+    def als_ortho3(H, Q, nals):
+        return Ref._block(Ref.run(3, H, Q, nals=nals), Q)[0]
+
+    @staticmethod
+    def getqhq_overlaps(H, Q, nals):
+        v = Ref.run(4, H, Q, nals=nals)
+        n = Q.nbloc
+        return (v[:n * n].reshape(n, n, order="F"),
+                v[n * n:2 * n * n].reshape(n, n, order="F"))
+
+    @staticmethod
+    def cycle(H, Q, npow, nals, update=1, eshift=0.0):
+        out, rest = Ref._block(Ref.run(5, H, Q, npow=npow, nals=nals,
+                                       update=update, eshift=eshift), Q)
+        out.eigv = rest[:Q.nbloc].copy()
+        return out
+
+    @staticmethod
+    def updatevecs2(H, Q, U, nals):
+        return Ref._block(Ref.run(6, H, Q, nals=nals, extra=U), Q)[0]
+
+    @staticmethod
+    def reduc_als(H, Q, nals):
+        """Q holds F (state 0) then G (state 1); returns reduced F."""
+        v = Ref.run(7, H, Q, nals=nals)
+        rF, nD = int(Q.ranks[0]), Q.base.shape[1]
+        return v[:rF].copy(), v[rF:rF + rF * nD].reshape(rF, nD).copy()
+
+    @staticmethod
+    def rayleigh(H, Q):
+        return Ref.run(8, H, Q)[:Q.nbloc].copy()
+
+    @staticmethod
+    def dsygv(H, Q, A, B):
+        n = Q.nbloc
+        v = Ref.run(9, H, Q, extra=np.concatenate([A.ravel(order="F"),
+                                                   B.ravel(order="F")]))
+        return v[:n].copy(), v[n:n + n * n].reshape(n, n, order="F")
