@@ -1,0 +1,115 @@
+/*
+ * C-ABI of the MI355X intertwined power/ALS library (libmlcp_hip.so).
+ *
+ * Drop-in exports.  The reference's Fortran host calls these as implicit
+ * externals (lower case, trailing underscore, every argument by address);
+ * GPUIterate.f90:78-80, 162-167, 273 are the call sites.  Fortran LOGICAL is
+ * 4 bytes, so `update` is read as int32 != 0.
+ *
+ *  intertwinecpp_   replaces intertwining.cu:98-176
+ *                   (called by GPU_Intertwine, GPUIterate.f90:89-187)
+ *  alscpp_          replaces intertwining.cu:180-202
+ *                   (called by GPU_ALS_AB, GPUIterate.f90:191-285)
+ *  intwdevicemem_   replaces intertwining.cu:31-94
+ *                   (called by Get_GPU_Memory, GPUIterate.f90:63-85)
+ *
+ * Semantics follow the reference's FP64 CPU path (lowmem=3): one call runs
+ * npow intertwined power sweeps on every state (ALS_POW3), ALS Gram-Schmidt
+ * (ALS_ORTHO_3) and, if update, the block generalized eigenproblem
+ * (GetQHQ3 + GetOverlaps + DSYGV) and the vector updates (ALS_SUMLCVEC_3).
+ * base/coef are overwritten with the new block, eigv with the eigenvalues
+ * (only if update).  On an error the host arrays are left unchanged, the
+ * message is printed on stderr and kept for mlcp_hip_last_error().
+ *
+ * Extensions (no counterpart in the reference) take plain pointers/sizes and
+ * return 0 on success: ragged ranks, a device-resident session for repeated
+ * cycles, and per-phase entry points used by the parity tests.
+ */
+#ifndef MLCP_HIP_H
+#define MLCP_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* intertwining.cu:98-104.  base(nD, rk*nbloc) column major, coef(rk*nbloc),
+ * nbas(ndof), opid(TxD), mptr(nops), Hcoef(TxD/ndof), mat(msize), eigv(nbloc).
+ * ncpu is accepted and ignored (the reference used it for OpenMP threads). */
+void intertwinecpp_(double *base, double *coef, int *nbas, const int *rk,
+                    const int *ndof, const int *nbloc, int *opid, int *mptr,
+                    double *Hcoef, double *mat, const int *TxD,
+                    const int *nops, const int *msize, double *eigv,
+                    const int *npow, const int *ncpu, const int *nals,
+                    const int *update, const int *ishift,
+                    const double *Eshift);
+
+/* intertwining.cu:180-202.  coef/base hold F (rank rF) then G (rank rG);
+ * F's terms are overwritten with the ALS fit of G after nals sweeps
+ * (reduc_ALS semantics, REDUCTIONML.f90:593-733). */
+void alscpp_(double *coef, double *base, int *nbas, const int *ndof,
+             const int *rF, const int *rG, const int *nals);
+
+/* intertwining.cu:31-94.  Prints the device memory this library needs. */
+void intwdevicemem_(const int *rk, const int *ndof, const int *nbloc,
+                    const int *TxD, const int *nops, const int *msize,
+                    const int *ncpu, const int *update, const int *nbas);
+
+/* ---- extensions ---- */
+
+/* Last error message of the calling thread ("" if none). */
+const char *mlcp_hip_last_error(void);
+
+/* Same as intertwinecpp_ with per-state ranks (ranks[nbloc]); T terms. */
+int mlcp_hip_intertwine(double *base, double *coef, const int *nbas, int ndof,
+                        int nbloc, const int *ranks, const int *opid,
+                        const int *mptr, const double *hcoef,
+                        const double *mat, int T, int nops, int msize,
+                        double *eigv, int npow, int nals, int update,
+                        int ishift, double eshift);
+
+/* Device-resident session: the block and the term list stay in HBM across
+ * cycles. */
+typedef struct mlcp_session mlcp_session;
+int mlcp_hip_session_create(mlcp_session **out, const double *base,
+                            const double *coef, const int *nbas, int ndof,
+                            int nbloc, const int *ranks, const int *opid,
+                            const int *mptr, const double *hcoef,
+                            const double *mat, int T, int nops, int msize);
+/* One intertwinecpp_ cycle on the resident block; eigv in/out. */
+int mlcp_hip_session_cycle(mlcp_session *s, double *eigv, int npow, int nals,
+                           int update, int ishift, double eshift);
+int mlcp_hip_session_download(mlcp_session *s, double *base, double *coef);
+/* Replaces the resident block (same ranks). */
+int mlcp_hip_session_upload(mlcp_session *s, const double *base,
+                            const double *coef);
+/* Milliseconds spent in each phase by the last cycle (HIP events):
+ * [power, gram-schmidt, reduced H*Q, inner products + eigensolve, updates]. */
+int mlcp_hip_session_phase_ms(mlcp_session *s, double *ms5);
+int mlcp_hip_session_destroy(mlcp_session *s);
+
+/* Single phases on the resident block, for parity tests.
+ *  power:    npow ALS_POW3 sweeps on every state
+ *  ortho:    ALS_ORTHO_3 with nals sweeps
+ *  prodhv:   PRODHV_ALS3 (ishift 0) on every state; the reduced H*Q block
+ *            is returned in hbase/hcoef (block layout), the block is kept
+ *  gep:      QHQ/S (upper triangles, column major nbloc x nbloc), the
+ *            eigenvectors U and eigenvalues w of the GEP
+ *  update:   UpdateVecs2 with the given U (nbloc x nbloc column major) */
+int mlcp_hip_session_power(mlcp_session *s, int npow, int ishift,
+                           double eshift);
+int mlcp_hip_session_ortho(mlcp_session *s, int nals);
+int mlcp_hip_session_prodhv(mlcp_session *s, int nals, double *hbase,
+                            double *hcoef);
+int mlcp_hip_session_gep(mlcp_session *s, int nals, double *QHQ, double *S,
+                         double *U, double *w);
+int mlcp_hip_session_update(mlcp_session *s, const double *U, int nals);
+
+/* Generalized eigenproblem alone (one workgroup on the GPU): A, B upper
+ * triangles (column major n x n); X eigenvectors, w ascending eigenvalues. */
+int mlcp_hip_gep(int n, const double *A, const double *B, double *X,
+                 double *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,106 @@
+// Device-side types and kernel entry points of the MI355X intertwined
+// power/ALS library.  Everything here is FP64 unless named otherwise.
+//
+// Factor pool layout in HBM ("mode-blocked"): for mode d the factors of every
+// CP term ("column" c) are stored contiguously,
+//     fac[moff[d] + c * n_d + i],   i < n_d,
+// so a mode-k step of every state reads and writes one contiguous slab.  The
+// pool holds three column regions of Ntot columns each: the block being
+// iterated (Q), the reduced H*Q vectors or other work vectors (W), and a copy
+// of the old block (O) used by the vector updates.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MLCP_MAXD 16      // modes per call
+#define MLCP_RMAX 64      // CP rank per state
+#define MLCP_NMAX 128     // basis functions per mode
+#define MLCP_BMAX 256     // states per block
+#define MLCP_NT 256       // threads per workgroup
+
+struct Pool {
+  int ndof;
+  int nbas[MLCP_MAXD];
+  long moff[MLCP_MAXD];
+  double* fac;
+  double* coef;
+};
+
+__device__ __forceinline__ double* colp(const Pool& p, int d, int c) {
+  return p.fac + p.moff[d] + (long)c * p.nbas[d];
+}
+
+// Operator-ALS (power method and reduced H*v), batched over states.
+struct OpArgs {
+  Pool p;
+  int B;                 // states in the batch
+  const int* col;        // first pool column of F per state
+  const int* ocol;       // first pool column of Fold per state
+  const int* R;          // rank per state
+  const double* pen;     // penalty per state
+  int rstr;              // stride of an R x R cache entry (Rmax^2)
+  int ystr;              // stride of an n x R cache entry (nmax*Rmax)
+  int nops;
+  const double* opd;     // dense operator matrices (n_d x n_d each)
+  const long* opoff;     // offset of each operator in opd
+  const int* opcls;      // 0 identity, 1 diagonal, 2 dense
+  const int* opmode;     // mode of each operator
+  int T;
+  const int* top;        // term operators, 0-based op index, [t*ndof + d]
+  const double* tcoef;   // term coefficients
+  // per-mode slot tables: the distinct operators used on mode k and the
+  // terms sorted by their mode-k operator
+  const int* kslot_off;  // [ndof+1] offsets into kslot_op / kslot_start
+  const int* kslot_op;   // op index of slot
+  const int* kslot_beg;  // [slot] -> start into korder (one past: next slot)
+  const int* kslot_end;
+  const int* korder;     // term ids sorted by slot, per mode (offset kord_off)
+  const int* kord_off;
+  double* M;             // [B][nops][rstr]  M(i,l) = <F_l, h Fold_i>
+  double* G;             // [B][ndof][rstr]  G(i,l) = <F_l, F_i>
+  double* Y;             // [B][nops][ystr]  Y(:,i) = h Fold_i
+  double* W;             // [B][maxslots][rstr]
+  double* BB;            // [B][rstr]
+  int maxslots;
+  int* err;              // device error word
+};
+
+// Linear-combination ALS (vector updates, rank reduction), batched over
+// targets.  Target t has factors F (pool columns col[t]..+R[t]) and is fitted
+// to sum_c w[t][c] * O_c over the ncold "old" columns starting at ocol0.
+struct LcArgs {
+  Pool p;
+  int B;                 // targets
+  const int* col;
+  const int* R;
+  const double* pen;
+  int ocol0;             // first old column
+  int ncold;             // number of old columns
+  const double* wcol;    // [B][ncold] weight per (target, old column)
+  int rstr;
+  int xr;                // row stride of X (max rank of the targets)
+  int nchunk;            // old-column chunks of LC_CHUNK
+  double* X;             // [B][ndof][xr * ncold]  X(i,c) = <F_i^d, O_c^d>
+  double* G;             // [B][ndof][rstr]
+  double* part;          // [B][nchunk][Rmax * nmax] partial right-hand sides
+  int pstr;              // Rmax * nmax
+  int* err;
+};
+
+#define LC_CHUNK 64
+
+// Gram-Schmidt (ALS_ORTHO_3) step for one target state iv.
+struct GsArgs {
+  Pool p;
+  int iv;
+  const int* col;        // first column per state
+  const int* R;          // rank per state
+  const double* pen;     // penalty of state iv (device scalar)
+  int rstr;
+  int pstr;
+  double* X;             // [B][ndof][rstr]  X_l(j,i) = <Q_l,j, F_i>
+  double* G;             // [ndof][rstr]     own Gram
+  double* part;          // [iv+1][pstr]
+  double* bbs;           // [rstr] prod_{d!=k} G_d
+  int* err;
+};

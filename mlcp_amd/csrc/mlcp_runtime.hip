@@ -1,0 +1,971 @@
+// Host runtime and C-ABI of the MI355X intertwined power/ALS library.
+//
+// One translation unit: the kernels are included so that hipcc builds a
+// single shared object without relocatable device code.
+//
+// A call of intertwinecpp_ (intertwining.cu:98-176 in the reference) becomes:
+//   upload block (reference layout) -> mode-blocked pool in HBM
+//   power phase      npow sweeps, all states batched     (ALS_POW3)
+//   Gram-Schmidt     sequential over states              (ALS_ORTHO_3)
+//   if update:
+//     reduced H*Q    nals sweeps, all states batched     (PRODHV_ALS3)
+//     QHQ, S         pairwise CP inner products          (GetQHQ3/GetOverlaps)
+//     GEP            one-workgroup FP64 eigensolver      (DSYGV semantics)
+//     updates        nals sweeps, all targets batched    (ALS_SUMLCVEC_3)
+//   download block.
+// Everything runs on one non-default stream of default priority.
+#include "mlcp_kernels.hip"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mlcp_hip.h"
+
+static thread_local std::string g_err;
+
+static int fail(const std::string& m) {
+  g_err = m;
+  fprintf(stderr, "mlcp_hip: %s\n", m.c_str());
+  return -1;
+}
+
+#define HCHK(x)                                                            \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess)                                                  \
+      return fail(std::string(#x) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int need(size_t cnt) {
+    if (cnt <= n) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (cnt == 0) return 0;
+    HCHK(hipMalloc((void**)&p, cnt * sizeof(T)));
+    n = cnt;
+    return 0;
+  }
+  int put(const std::vector<T>& v, hipStream_t st) {
+    if (need(v.size() ? v.size() : 1)) return -1;
+    // synchronous with respect to the host vector, which may be a temporary
+    if (!v.empty()) {
+      HCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T),
+                          hipMemcpyHostToDevice, st));
+      HCHK(hipStreamSynchronize(st));
+    }
+    return 0;
+  }
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// Dynamic LDS limit per workgroup: the device's per-block maximum less a
+// margin for the kernels' static LDS.
+static size_t g_lds_limit = 64 * 1024;
+
+static int setup_kernels() {
+  static bool done = false;
+  if (done) return 0;
+  int dev = 0, v = 0;
+  HCHK(hipGetDevice(&dev));
+  HCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                             dev));
+  size_t lim = (size_t)v > 4096 ? (size_t)v - 4096 : (size_t)v;
+  const void* ks[] = {(const void*)k_op_refresh, (const void*)k_op_solve,
+                      (const void*)k_gs_terms,   (const void*)k_gs_solve,
+                      (const void*)k_lc_rhs,     (const void*)k_lc_solve};
+  for (const void* k : ks)
+    HCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lim));
+  g_lds_limit = lim;
+  done = true;
+  return 0;
+}
+
+// Host-side check of a launch's dynamic LDS request.
+static int lds_ok(size_t bytes, const char* kernel) {
+  if (bytes > g_lds_limit)
+    return fail(std::string(kernel) + ": needs " + std::to_string(bytes) +
+                " bytes of LDS, the device allows " +
+                std::to_string(g_lds_limit));
+  return 0;
+}
+
+struct mlcp_session {
+  int ndof = 0, nbloc = 0, nD = 0, Ntot = 0, Rmax = 0, nmax = 0;
+  std::vector<int> nbas, ranks, roff, off;
+  hipStream_t st = nullptr;
+  Pool pool;
+  DBuf<double> fac, coef;
+  DBuf<int> err;
+  // term list
+  int T = 0, nops = 0, maxslots = 1;
+  std::vector<int> opmode, opcls, top, kslot_off, kslot_op, kslot_beg,
+      kslot_end, korder, kord_off;
+  std::vector<long> opoff;
+  std::vector<std::vector<int>> mode_ops;
+  DBuf<double> opd, tcoef;
+  DBuf<long> d_opoff;
+  DBuf<int> d_opmode, d_opcls, d_top, d_kslot_off, d_kslot_op, d_kslot_beg,
+      d_kslot_end, d_korder, d_kord_off;
+  // per-state tables
+  DBuf<int> d_colQ, d_colW, d_R, d_colstate;
+  DBuf<double> d_pen;
+  // operator-ALS caches
+  DBuf<double> M, G, Y, W, BB;
+  DBuf<int2> jobs_all, jobs_had;
+  std::vector<DBuf<int2>> jobs_mode;
+  std::vector<int> njobs_mode;
+  int njobs_all = 0, njobs_had = 0;
+  // Gram-Schmidt
+  DBuf<double> gsX, gsG, gsPart, gsBbs;
+  // linear-combination ALS
+  DBuf<double> lcX, lcG, lcPart, lcW;
+  DBuf<int2> lcJobs;
+  // GEP
+  DBuf<double> QHQ, S, U, eig, ws;
+  DBuf<PairJob> pj;
+  int npj = 0;
+  hipEvent_t ev[6] = {};
+  double phase_ms[5] = {0, 0, 0, 0, 0};
+  ~mlcp_session() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+// --------------------------------------------------------------------------
+// term list: bookkeeping from the reference HopList (ModeH.f90:909-1068)
+
+static int build_terms(mlcp_session* s, const int* opid, const int* mptr,
+                       const double* hcoef, const double* mat, int T,
+                       int nops, int msize) {
+  int D = s->ndof;
+  if (T < 1 || nops < 1 || msize < 1) return fail("empty term list");
+  s->T = T;
+  s->nops = nops;
+  s->opmode.assign(nops, -1);
+  for (int t = 0; t < T; t++)
+    for (int d = 0; d < D; d++) {
+      int op = opid[t * D + d];
+      if (op < 1 || op > nops)
+        return fail("opid out of range at term " + std::to_string(t + 1));
+      int& m = s->opmode[op - 1];
+      if (m >= 0 && m != d)
+        return fail("operator " + std::to_string(op) + " used on two modes");
+      m = d;
+    }
+  s->opoff.assign(nops, 0);
+  s->opcls.assign(nops, 2);
+  std::vector<double> dense;
+  for (int op = 0; op < nops; op++) {
+    int d = s->opmode[op];
+    if (d < 0) continue;
+    int n = s->nbas[d];
+    long lo = (long)mptr[op] - 1;
+    if (lo < 0 || lo + (long)n * (n + 1) / 2 > msize)
+      return fail("operator " + std::to_string(op + 1) + " matrix outside mat");
+    s->opoff[op] = (long)dense.size();
+    dense.resize(dense.size() + (size_t)n * n);
+    double* h = dense.data() + s->opoff[op];
+    bool offzero = true, unit = true;
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i <= j; i++) {
+        double v = mat[lo + i + (long)j * (j + 1) / 2];
+        h[i * n + j] = v;
+        h[j * n + i] = v;
+        if (i != j && v != 0.0) offzero = false;
+        if (i == j && v != 1.0) unit = false;
+      }
+    s->opcls[op] = offzero ? (unit ? 0 : 1) : 2;
+  }
+  s->top.assign((size_t)T * D, 0);
+  for (int i = 0; i < T * D; i++) s->top[i] = opid[i] - 1;
+  // per-mode slots: distinct operators on mode k in order of first use
+  s->mode_ops.assign(D, {});
+  s->kslot_off.assign(D + 1, 0);
+  s->kslot_op.clear();
+  s->kslot_beg.clear();
+  s->kslot_end.clear();
+  s->korder.clear();
+  s->kord_off.assign(D + 1, 0);
+  s->maxslots = 1;
+  for (int k = 0; k < D; k++) {
+    std::vector<int>& ops = s->mode_ops[k];
+    std::vector<int> slot(T);
+    for (int t = 0; t < T; t++) {
+      int op = s->top[t * D + k];
+      auto it = std::find(ops.begin(), ops.end(), op);
+      slot[t] = (int)(it - ops.begin());
+      if (it == ops.end()) ops.push_back(op);
+    }
+    s->kslot_off[k] = (int)s->kslot_op.size();
+    s->kord_off[k] = (int)s->korder.size();
+    int base = 0;
+    for (int u = 0; u < (int)ops.size(); u++) {
+      s->kslot_op.push_back(ops[u]);
+      s->kslot_beg.push_back(base);
+      for (int t = 0; t < T; t++)
+        if (slot[t] == u) {
+          s->korder.push_back(t);
+          base++;
+        }
+      s->kslot_end.push_back(base);
+    }
+    s->maxslots = std::max(s->maxslots, (int)ops.size());
+  }
+  s->kslot_off[D] = (int)s->kslot_op.size();
+  s->kord_off[D] = (int)s->korder.size();
+  std::vector<double> tc(hcoef, hcoef + T);
+  hipStream_t st = s->st;
+  if (s->opd.put(dense, st) || s->tcoef.put(tc, st) ||
+      s->d_opoff.put(s->opoff, st) || s->d_opmode.put(s->opmode, st) ||
+      s->d_opcls.put(s->opcls, st) || s->d_top.put(s->top, st) ||
+      s->d_kslot_off.put(s->kslot_off, st) ||
+      s->d_kslot_op.put(s->kslot_op, st) ||
+      s->d_kslot_beg.put(s->kslot_beg, st) ||
+      s->d_kslot_end.put(s->kslot_end, st) ||
+      s->d_korder.put(s->korder, st) || s->d_kord_off.put(s->kord_off, st))
+    return -1;
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// session setup
+
+static int session_alloc(mlcp_session* s) {
+  int D = s->ndof, B = s->nbloc;
+  hipStream_t st = s->st;
+  int rstr = s->Rmax * s->Rmax;
+  int ystr = s->nmax * s->Rmax;
+  // pool: three regions of Ntot columns
+  int cap = 3 * s->Ntot;
+  if (s->fac.need((size_t)cap * s->nD) || s->coef.need(cap) ||
+      s->err.need(1))
+    return -1;
+  s->pool.ndof = D;
+  long mo = 0;
+  for (int d = 0; d < D; d++) {
+    s->pool.nbas[d] = s->nbas[d];
+    s->pool.moff[d] = mo;
+    mo += (long)cap * s->nbas[d];
+  }
+  s->pool.fac = s->fac.p;
+  s->pool.coef = s->coef.p;
+  std::vector<int> colQ(B), colW(B), colstate(s->Ntot);
+  for (int i = 0; i < B; i++) {
+    colQ[i] = s->roff[i];
+    colW[i] = s->Ntot + s->roff[i];
+    for (int r = 0; r < s->ranks[i]; r++) colstate[s->roff[i] + r] = i;
+  }
+  if (s->d_colQ.put(colQ, st) || s->d_colW.put(colW, st) ||
+      s->d_R.put(s->ranks, st) || s->d_colstate.put(colstate, st) ||
+      s->d_pen.need(B))
+    return -1;
+  // operator-ALS caches
+  if (s->M.need((size_t)B * s->nops * rstr) ||
+      s->G.need((size_t)B * D * rstr) ||
+      s->Y.need((size_t)B * s->nops * ystr) ||
+      s->W.need((size_t)B * s->maxslots * rstr) || s->BB.need((size_t)B * rstr))
+    return -1;
+  // refresh jobs
+  s->jobs_mode.clear();
+  s->jobs_mode.resize(D);
+  s->njobs_mode.assign(D, 0);
+  std::vector<int2> all;
+  for (int d = 0; d < D; d++) {
+    std::vector<int2> j;
+    for (int i = 0; i < B; i++) {
+      for (int op : s->mode_ops[d]) j.push_back(make_int2(i, op));
+      j.push_back(make_int2(i, -(d + 1)));
+    }
+    s->njobs_mode[d] = (int)j.size();
+    if (s->jobs_mode[d].put(j, st)) return -1;
+    all.insert(all.end(), j.begin(), j.end());
+  }
+  s->njobs_all = (int)all.size();
+  if (s->jobs_all.put(all, st)) return -1;
+  std::vector<int2> had;
+  for (int i = 0; i < B; i++) {
+    int nt = (s->ranks[i] * s->ranks[i] + MLCP_NT - 1) / MLCP_NT;
+    for (int t = 0; t < nt; t++) had.push_back(make_int2(i, t));
+  }
+  s->njobs_had = (int)had.size();
+  if (s->jobs_had.put(had, st)) return -1;
+  // Gram-Schmidt
+  int pstr = s->Rmax * s->nmax;
+  if (s->gsX.need((size_t)B * D * rstr) || s->gsG.need((size_t)D * rstr) ||
+      s->gsPart.need((size_t)(B + 1) * pstr) || s->gsBbs.need(rstr))
+    return -1;
+  // GEP
+  std::vector<PairJob> pj;
+  for (int i = 0; i < B; i++)
+    for (int j = i; j < B; j++) {
+      // S(i,j) = <Q_j, Q_i>, QHQ(i,j) = <Q_j, HQ_i>
+      pj.push_back({s->roff[j], s->ranks[j], s->roff[i], s->ranks[i],
+                    (long)i + (long)j * B});
+      pj.push_back({s->roff[j], s->ranks[j], s->Ntot + s->roff[i],
+                    s->ranks[i], (long)B * B + i + (long)j * B});
+    }
+  s->npj = (int)pj.size();
+  if (s->pj.put(pj, st) || s->QHQ.need((size_t)2 * B * B) ||
+      s->U.need((size_t)B * B) || s->eig.need(B) ||
+      s->ws.need((size_t)4 * B * B + 2 * B + 8))
+    return -1;
+  for (auto& e : s->ev)
+    if (!e) HCHK(hipEventCreate(&e));
+  return 0;
+}
+
+// Largest dynamic LDS request of any kernel for these sizes, checked once.
+static int check_lds(int Rmax, int nmax) {
+  size_t r = Rmax, n = nmax, b = sizeof(double);
+  if (lds_ok(2 * n * r * b, "k_op_refresh") ||
+      lds_ok((r * n + r * r) * b, "k_op_solve/k_gs_solve/k_lc_solve") ||
+      lds_ok(r * r * b, "k_gs_terms") ||
+      lds_ok(((size_t)LC_CHUNK * r + (size_t)LC_CHUNK * n) * b, "k_lc_rhs"))
+    return -1;
+  return 0;
+}
+
+static OpArgs op_args(mlcp_session* s, bool work) {
+  OpArgs a;
+  a.p = s->pool;
+  a.B = s->nbloc;
+  a.col = work ? s->d_colW.p : s->d_colQ.p;
+  a.ocol = s->d_colQ.p;
+  a.R = s->d_R.p;
+  a.pen = s->d_pen.p;
+  a.rstr = s->Rmax * s->Rmax;
+  a.ystr = s->nmax * s->Rmax;
+  a.nops = s->nops;
+  a.opd = s->opd.p;
+  a.opoff = s->d_opoff.p;
+  a.opcls = s->d_opcls.p;
+  a.opmode = s->d_opmode.p;
+  a.T = s->T;
+  a.top = s->d_top.p;
+  a.tcoef = s->tcoef.p;
+  a.kslot_off = s->d_kslot_off.p;
+  a.kslot_op = s->d_kslot_op.p;
+  a.kslot_beg = s->d_kslot_beg.p;
+  a.kslot_end = s->d_kslot_end.p;
+  a.korder = s->d_korder.p;
+  a.kord_off = s->d_kord_off.p;
+  a.M = s->M.p;
+  a.G = s->G.p;
+  a.Y = s->Y.p;
+  a.W = s->W.p;
+  a.BB = s->BB.p;
+  a.maxslots = s->maxslots;
+  a.err = s->err.p;
+  return a;
+}
+
+static int check_err(mlcp_session* s, const char* phase) {
+  int e = 0;
+  HCHK(hipMemcpyAsync(&e, s->err.p, sizeof(int), hipMemcpyDeviceToHost,
+                      s->st));
+  HCHK(hipStreamSynchronize(s->st));
+  if (e) {
+    std::string m = std::string(phase) + ": ";
+    if (e & 1) m += "NaN or zero norm on update; ";
+    if (e & 2) m += "non-positive pivot in the ALS normal equations; ";
+    if (e & 4) m += "overlap matrix not positive definite; ";
+    return fail(m);
+  }
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// phases
+
+static size_t lds_refresh(mlcp_session* s, int d) {
+  return (size_t)2 * s->nbas[d] * s->Rmax * sizeof(double);
+}
+
+static int refresh_all(mlcp_session* s, const OpArgs& a, int do_y) {
+  size_t lds = 0;
+  for (int d = 0; d < s->ndof; d++) lds = std::max(lds, lds_refresh(s, d));
+  hipLaunchKernelGGL(k_op_refresh, dim3(s->njobs_all), dim3(MLCP_NT), lds,
+                     s->st, a, (const int2*)s->jobs_all.p, do_y);
+  HCHK(hipGetLastError());
+  return 0;
+}
+
+// Shared sweep loop of the power method and the reduced H*v.
+static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
+                  double eshift, bool renorm, bool do_y_each) {
+  OpArgs a = op_args(s, work);
+  int D = s->ndof;
+  hipLaunchKernelGGL(k_penalty, dim3((s->nbloc + 63) / 64), dim3(64), 0,
+                     s->st, s->pool, a.ocol, a.R, 1.e-10, s->d_pen.p,
+                     s->nbloc);
+  if (refresh_all(s, a, 1)) return -1;
+  for (int it = 0; it < nitn; it++) {
+    for (int k = 0; k < D; k++) {
+      hipLaunchKernelGGL(k_op_hadamard, dim3(s->njobs_had), dim3(MLCP_NT), 0,
+                         s->st, a, (const int2*)s->jobs_had.p, k);
+      size_t lds = ((size_t)s->Rmax * s->nbas[k] + (size_t)s->Rmax * s->Rmax) *
+                   sizeof(double);
+      hipLaunchKernelGGL(k_op_solve, dim3(s->nbloc), dim3(MLCP_NT), lds,
+                         s->st, a, k, ishift, eshift);
+      bool last = (it == nitn - 1 && k == D - 1);
+      if (!last || renorm) {
+        hipLaunchKernelGGL(k_op_refresh, dim3(s->njobs_mode[k]),
+                           dim3(MLCP_NT), lds_refresh(s, k), s->st, a,
+                           (const int2*)s->jobs_mode[k].p,
+                           do_y_each ? 1 : 0);
+      }
+      if (renorm && k == D - 1)
+        hipLaunchKernelGGL(k_op_renorm, dim3(s->nbloc), dim3(MLCP_NT), 0,
+                           s->st, a);
+    }
+  }
+  HCHK(hipGetLastError());
+  return 0;
+}
+
+static int phase_power(mlcp_session* s, int npow, int ishift, double eshift) {
+  if (npow < 1) return 0;
+  return op_als(s, false, npow, ishift, eshift, true, true);
+}
+
+static int phase_prodhv(mlcp_session* s, int nals) {
+  // F <- copy of Fold in the work region, then PRODHV_ALS3 sweeps
+  hipLaunchKernelGGL(k_copy_cols, dim3(256), dim3(256), 0, s->st, s->pool, 0,
+                     s->Ntot, s->Ntot);
+  if (nals < 1) return 0;
+  return op_als(s, true, nals, 0, 0.0, false, false);
+}
+
+static GsArgs gs_args(mlcp_session* s, int iv) {
+  GsArgs g;
+  g.p = s->pool;
+  g.iv = iv;
+  g.col = s->d_colQ.p;
+  g.R = s->d_R.p;
+  g.pen = s->d_pen.p + iv;
+  g.rstr = s->Rmax * s->Rmax;
+  g.pstr = s->Rmax * s->nmax;
+  g.X = s->gsX.p;
+  g.G = s->gsG.p;
+  g.part = s->gsPart.p;
+  g.bbs = s->gsBbs.p;
+  g.err = s->err.p;
+  return g;
+}
+
+static int phase_ortho(mlcp_session* s, int nals) {
+  if (nals < 1) return 0;
+  int D = s->ndof;
+  hipLaunchKernelGGL(k_normcoef, dim3(1), dim3(MLCP_NT), 0, s->st, s->pool,
+                     s->roff[0], s->ranks[0]);
+  hipLaunchKernelGGL(k_penalty, dim3((s->nbloc + 63) / 64), dim3(64), 0,
+                     s->st, s->pool, s->d_colQ.p, s->d_R.p, 1.e-10,
+                     s->d_pen.p, s->nbloc);
+  size_t lds_t = (size_t)s->Rmax * s->Rmax * sizeof(double);
+  for (int iv = 1; iv < s->nbloc; iv++) {
+    GsArgs g = gs_args(s, iv);
+    int kprev = -1;
+    for (int it = 0; it < nals; it++)
+      for (int k = 0; k < D; k++) {
+        hipLaunchKernelGGL(k_gs_terms, dim3(iv + 1), dim3(MLCP_NT), lds_t,
+                           s->st, g, k, kprev);
+        size_t lds = ((size_t)s->Rmax * s->nbas[k] +
+                      (size_t)s->Rmax * s->Rmax) * sizeof(double);
+        hipLaunchKernelGGL(k_gs_solve, dim3(1), dim3(MLCP_NT), lds, s->st, g,
+                           k);
+        kprev = k;
+      }
+  }
+  HCHK(hipGetLastError());
+  return 0;
+}
+
+static int phase_gep(mlcp_session* s) {
+  int B = s->nbloc;
+  hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, s->st, s->QHQ.p,
+                     2L * B * B, 0.0);
+  hipLaunchKernelGGL(k_pair_inner, dim3(s->npj), dim3(MLCP_NT), 0, s->st,
+                     s->pool, (const PairJob*)s->pj.p, s->QHQ.p);
+  // QHQ at QHQ.p + B*B, S at QHQ.p
+  hipLaunchKernelGGL(k_gep, dim3(1), dim3(1024), 0, s->st,
+                     (const double*)(s->QHQ.p + (size_t)B * B),
+                     (const double*)s->QHQ.p, B, s->ws.p, s->U.p, s->eig.p,
+                     s->err.p);
+  HCHK(hipGetLastError());
+  return 0;
+}
+
+static LcArgs lc_args(mlcp_session* s, int B, const int* col, const int* R,
+                      int xr, int ocol0, int ncold) {
+  LcArgs a;
+  a.p = s->pool;
+  a.B = B;
+  a.col = col;
+  a.R = R;
+  a.pen = s->d_pen.p;
+  a.ocol0 = ocol0;
+  a.ncold = ncold;
+  a.wcol = s->lcW.p;
+  a.rstr = s->Rmax * s->Rmax;
+  a.xr = xr;
+  a.nchunk = (ncold + LC_CHUNK - 1) / LC_CHUNK;
+  a.X = s->lcX.p;
+  a.G = s->lcG.p;
+  a.part = s->lcPart.p;
+  a.pstr = s->Rmax * s->nmax;
+  a.err = s->err.p;
+  return a;
+}
+
+// Linear-combination ALS sweeps on the prepared LcArgs (weights in lcW,
+// penalties in d_pen).
+static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
+  int D = s->ndof, B = a.B;
+  std::vector<int2> jobs;
+  for (int t = 0; t < B; t++)
+    for (int c = 0; c < a.nchunk; c++) jobs.push_back(make_int2(t, c));
+  if (s->lcJobs.put(jobs, s->st)) return -1;
+  int nj = (int)jobs.size();
+  for (int d = 0; d < D; d++)
+    hipLaunchKernelGGL(k_lc_refresh, dim3(nj), dim3(MLCP_NT), 0, s->st, a,
+                       (const int2*)s->lcJobs.p, d);
+  for (int it = 0; it < nitn; it++)
+    for (int k = 0; k < D; k++) {
+      size_t lds_r = ((size_t)LC_CHUNK * s->Rmax +
+                      (size_t)LC_CHUNK * s->nbas[k]) * sizeof(double);
+      hipLaunchKernelGGL(k_lc_rhs, dim3(nj), dim3(MLCP_NT), lds_r, s->st, a,
+                         (const int2*)s->lcJobs.p, k);
+      size_t lds_s = ((size_t)s->Rmax * s->nbas[k] +
+                      (size_t)s->Rmax * s->Rmax) * sizeof(double);
+      hipLaunchKernelGGL(k_lc_solve, dim3(B), dim3(MLCP_NT), lds_s, s->st, a,
+                         k);
+      if (!(it == nitn - 1 && k == D - 1))
+        hipLaunchKernelGGL(k_lc_refresh, dim3(nj), dim3(MLCP_NT), 0, s->st, a,
+                           (const int2*)s->lcJobs.p, k);
+    }
+  if (final_norm)
+    hipLaunchKernelGGL(k_lc_final, dim3(B), dim3(MLCP_NT), 0, s->st, a);
+  HCHK(hipGetLastError());
+  return 0;
+}
+
+// UpdateVecs2 (BlockUtils.f90:479-516): Q_i <- ALS fit of sum_l U(l,i) Qold_l
+// with Q_i as the starting guess.  U is a device pointer (B x B, col major).
+static int phase_update(mlcp_session* s, const double* Ud, int nals) {
+  if (nals < 1) return 0;
+  int B = s->nbloc, N = s->Ntot, D = s->ndof;
+  // old block -> region O
+  hipLaunchKernelGGL(k_copy_cols, dim3(256), dim3(256), 0, s->st, s->pool, 0,
+                     2 * N, N);
+  if (s->lcX.need((size_t)B * D * s->Rmax * N) ||
+      s->lcG.need((size_t)B * D * s->Rmax * s->Rmax) ||
+      s->lcPart.need((size_t)B * ((N + LC_CHUNK - 1) / LC_CHUNK) * s->Rmax *
+                     s->nmax) ||
+      s->lcW.need((size_t)B * N))
+    return -1;
+  hipLaunchKernelGGL(k_lc_weights, dim3(256), dim3(256), 0, s->st, s->lcW.p,
+                     Ud, B, (const int*)s->d_colstate.p, N, B);
+  hipLaunchKernelGGL(k_penalty, dim3((B + 63) / 64), dim3(64), 0, s->st,
+                     s->pool, s->d_colQ.p, s->d_R.p, 1.e-10, s->d_pen.p, B);
+  LcArgs a = lc_args(s, B, s->d_colQ.p, s->d_R.p, s->Rmax, 2 * N, N);
+  return lc_als(s, a, nals, true);
+}
+
+static int run_cycle(mlcp_session* s, double* eigv, int npow, int nals,
+                     int update, int ishift, double eshift) {
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s->st, s->err.p, 1L, 0);
+  HCHK(hipEventRecord(s->ev[0], s->st));
+  if (phase_power(s, npow, ishift, eshift)) return -1;
+  HCHK(hipEventRecord(s->ev[1], s->st));
+  if (phase_ortho(s, nals)) return -1;
+  HCHK(hipEventRecord(s->ev[2], s->st));
+  if (update) {
+    if (phase_prodhv(s, nals)) return -1;
+    HCHK(hipEventRecord(s->ev[3], s->st));
+    if (phase_gep(s)) return -1;
+    HCHK(hipEventRecord(s->ev[4], s->st));
+    if (phase_update(s, s->U.p, nals)) return -1;
+    HCHK(hipEventRecord(s->ev[5], s->st));
+  }
+  if (check_err(s, "intertwine cycle")) return -1;
+  float ms;
+  for (int i = 0; i < 5; i++) s->phase_ms[i] = 0.0;
+  for (int i = 0; i < (update ? 5 : 2); i++) {
+    HCHK(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+    s->phase_ms[i] = ms;
+  }
+  if (update)
+    HCHK(hipMemcpy(eigv, s->eig.p, sizeof(double) * s->nbloc,
+                   hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// C-ABI
+
+static int validate_dims(const int* nbas, int ndof, int nbloc,
+                         const int* ranks) {
+  if (ndof < 1 || ndof > MLCP_MAXD)
+    return fail("ndof must be in 1.." + std::to_string(MLCP_MAXD));
+  if (nbloc < 1 || nbloc > MLCP_BMAX)
+    return fail("nbloc must be in 1.." + std::to_string(MLCP_BMAX));
+  for (int d = 0; d < ndof; d++)
+    if (nbas[d] < 1 || nbas[d] > MLCP_NMAX)
+      return fail("nbas must be in 1.." + std::to_string(MLCP_NMAX));
+  for (int i = 0; i < nbloc; i++)
+    if (ranks[i] < 1 || ranks[i] > MLCP_RMAX)
+      return fail("rank must be in 1.." + std::to_string(MLCP_RMAX));
+  return 0;
+}
+
+extern "C" const char* mlcp_hip_last_error(void) { return g_err.c_str(); }
+
+extern "C" int mlcp_hip_session_create(mlcp_session** out, const double* base,
+                                       const double* coef, const int* nbas,
+                                       int ndof, int nbloc, const int* ranks,
+                                       const int* opid, const int* mptr,
+                                       const double* hcoef, const double* mat,
+                                       int T, int nops, int msize) {
+  g_err.clear();
+  *out = nullptr;
+  if (validate_dims(nbas, ndof, nbloc, ranks)) return -1;
+  if (setup_kernels()) return -1;
+  mlcp_session* s = new mlcp_session();
+  s->ndof = ndof;
+  s->nbloc = nbloc;
+  s->nbas.assign(nbas, nbas + ndof);
+  s->ranks.assign(ranks, ranks + nbloc);
+  s->roff.assign(nbloc + 1, 0);
+  for (int i = 0; i < nbloc; i++) s->roff[i + 1] = s->roff[i] + ranks[i];
+  s->Ntot = s->roff[nbloc];
+  s->off.assign(ndof + 1, 0);
+  for (int d = 0; d < ndof; d++) s->off[d + 1] = s->off[d] + nbas[d];
+  s->nD = s->off[ndof];
+  s->Rmax = *std::max_element(ranks, ranks + nbloc);
+  s->nmax = *std::max_element(nbas, nbas + ndof);
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return fail("hipStreamCreate failed");
+  }
+  if (check_lds(s->Rmax, s->nmax) ||
+      build_terms(s, opid, mptr, hcoef, mat, T, nops, msize) ||
+      session_alloc(s) || mlcp_hip_session_upload(s, base, coef)) {
+    std::string m = g_err;
+    delete s;
+    g_err = m;
+    return -1;
+  }
+  *out = s;
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_upload(mlcp_session* s, const double* base,
+                                       const double* coef) {
+  size_t nb = (size_t)s->Ntot * s->nD;
+  DBuf<double> stage;
+  if (stage.need(nb + s->Ntot)) return -1;
+  HCHK(hipMemcpyAsync(stage.p, base, nb * sizeof(double),
+                      hipMemcpyHostToDevice, s->st));
+  HCHK(hipMemcpyAsync(stage.p + nb, coef, s->Ntot * sizeof(double),
+                      hipMemcpyHostToDevice, s->st));
+  hipLaunchKernelGGL(k_to_pool, dim3(512), dim3(256), 0, s->st, stage.p,
+                     stage.p + nb, s->pool, 0, s->Ntot, s->nD);
+  HCHK(hipGetLastError());
+  HCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+static int download_region(mlcp_session* s, int c0, double* base,
+                           double* coef) {
+  size_t nb = (size_t)s->Ntot * s->nD;
+  DBuf<double> stage;
+  if (stage.need(nb + s->Ntot)) return -1;
+  hipLaunchKernelGGL(k_from_pool, dim3(512), dim3(256), 0, s->st, stage.p,
+                     stage.p + nb, s->pool, c0, s->Ntot, s->nD);
+  HCHK(hipGetLastError());
+  HCHK(hipMemcpyAsync(base, stage.p, nb * sizeof(double),
+                      hipMemcpyDeviceToHost, s->st));
+  HCHK(hipMemcpyAsync(coef, stage.p + nb, s->Ntot * sizeof(double),
+                      hipMemcpyDeviceToHost, s->st));
+  HCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_download(mlcp_session* s, double* base,
+                                         double* coef) {
+  g_err.clear();
+  return download_region(s, 0, base, coef);
+}
+
+extern "C" int mlcp_hip_session_cycle(mlcp_session* s, double* eigv, int npow,
+                                      int nals, int update, int ishift,
+                                      double eshift) {
+  g_err.clear();
+  return run_cycle(s, eigv, npow, nals, update, ishift, eshift);
+}
+
+extern "C" int mlcp_hip_session_phase_ms(mlcp_session* s, double* ms5) {
+  for (int i = 0; i < 5; i++) ms5[i] = s->phase_ms[i];
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_destroy(mlcp_session* s) {
+  if (s) {
+    (void)hipStreamSynchronize(s->st);
+    delete s;
+  }
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_power(mlcp_session* s, int npow, int ishift,
+                                      double eshift) {
+  g_err.clear();
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s->st, s->err.p, 1L, 0);
+  if (phase_power(s, npow, ishift, eshift)) return -1;
+  return check_err(s, "power");
+}
+
+extern "C" int mlcp_hip_session_ortho(mlcp_session* s, int nals) {
+  g_err.clear();
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s->st, s->err.p, 1L, 0);
+  if (phase_ortho(s, nals)) return -1;
+  return check_err(s, "gram-schmidt");
+}
+
+extern "C" int mlcp_hip_session_prodhv(mlcp_session* s, int nals,
+                                       double* hbase, double* hcoef) {
+  g_err.clear();
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s->st, s->err.p, 1L, 0);
+  if (phase_prodhv(s, nals) || check_err(s, "reduced H*Q")) return -1;
+  return download_region(s, s->Ntot, hbase, hcoef);
+}
+
+extern "C" int mlcp_hip_session_gep(mlcp_session* s, int nals, double* QHQ,
+                                    double* S, double* U, double* w) {
+  g_err.clear();
+  int B = s->nbloc;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s->st, s->err.p, 1L, 0);
+  if (phase_prodhv(s, nals) || phase_gep(s) || check_err(s, "gep")) return -1;
+  HCHK(hipMemcpy(S, s->QHQ.p, sizeof(double) * B * B, hipMemcpyDeviceToHost));
+  HCHK(hipMemcpy(QHQ, s->QHQ.p + (size_t)B * B, sizeof(double) * B * B,
+                 hipMemcpyDeviceToHost));
+  HCHK(hipMemcpy(U, s->U.p, sizeof(double) * B * B, hipMemcpyDeviceToHost));
+  HCHK(hipMemcpy(w, s->eig.p, sizeof(double) * B, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_update(mlcp_session* s, const double* U,
+                                       int nals) {
+  g_err.clear();
+  int B = s->nbloc;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s->st, s->err.p, 1L, 0);
+  HCHK(hipMemcpyAsync(s->U.p, U, sizeof(double) * B * B,
+                      hipMemcpyHostToDevice, s->st));
+  if (phase_update(s, s->U.p, nals)) return -1;
+  return check_err(s, "update");
+}
+
+extern "C" int mlcp_hip_gep(int n, const double* A, const double* Bm,
+                            double* X, double* w) {
+  g_err.clear();
+  if (n < 1 || n > MLCP_BMAX) return fail("gep size out of range");
+  DBuf<double> d;
+  DBuf<int> e;
+  size_t nn = (size_t)n * n;
+  if (d.need(2 * nn + 4 * nn + 2 * n + 8 + nn + n) || e.need(1)) return -1;
+  double *dA = d.p, *dB = d.p + nn, *ws = d.p + 2 * nn,
+         *dX = ws + 4 * nn + 2 * n + 8, *dw = dX + nn;
+  HCHK(hipMemcpy(dA, A, nn * sizeof(double), hipMemcpyHostToDevice));
+  HCHK(hipMemcpy(dB, Bm, nn * sizeof(double), hipMemcpyHostToDevice));
+  HCHK(hipMemset(e.p, 0, sizeof(int)));
+  hipLaunchKernelGGL(k_gep, dim3(1), dim3(1024), 0, 0, dA, dB, n, ws, dX, dw,
+                     e.p);
+  HCHK(hipGetLastError());
+  int err = 0;
+  HCHK(hipMemcpy(&err, e.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (err) return fail("gep: overlap matrix not positive definite");
+  HCHK(hipMemcpy(X, dX, nn * sizeof(double), hipMemcpyDeviceToHost));
+  HCHK(hipMemcpy(w, dw, n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int mlcp_hip_intertwine(double* base, double* coef,
+                                   const int* nbas, int ndof, int nbloc,
+                                   const int* ranks, const int* opid,
+                                   const int* mptr, const double* hcoef,
+                                   const double* mat, int T, int nops,
+                                   int msize, double* eigv, int npow, int nals,
+                                   int update, int ishift, double eshift) {
+  mlcp_session* s = nullptr;
+  if (mlcp_hip_session_create(&s, base, coef, nbas, ndof, nbloc, ranks, opid,
+                              mptr, hcoef, mat, T, nops, msize))
+    return -1;
+  std::vector<double> ev(eigv, eigv + nbloc);
+  int rc = run_cycle(s, ev.data(), npow, nals, update, ishift, eshift);
+  if (!rc) {
+    // write back only on success, so that errors leave the inputs intact
+    std::vector<double> b((size_t)s->Ntot * s->nD), c(s->Ntot);
+    rc = download_region(s, 0, b.data(), c.data());
+    if (!rc) {
+      memcpy(base, b.data(), b.size() * sizeof(double));
+      memcpy(coef, c.data(), c.size() * sizeof(double));
+      if (update) memcpy(eigv, ev.data(), nbloc * sizeof(double));
+    }
+  }
+  std::string m = g_err;
+  mlcp_hip_session_destroy(s);
+  g_err = m;
+  return rc;
+}
+
+extern "C" void intertwinecpp_(double* base, double* coef, int* nbas,
+                               const int* rk, const int* ndof,
+                               const int* nbloc, int* opid, int* mptr,
+                               double* Hcoef, double* mat, const int* TxD,
+                               const int* nops, const int* msize, double* eigv,
+                               const int* npow, const int* ncpu,
+                               const int* nals, const int* update,
+                               const int* ishift, const double* Eshift) {
+  (void)ncpu;
+  if (*ndof < 1 || *TxD % *ndof != 0) {
+    fail("TxD must be a multiple of ndof");
+    return;
+  }
+  if (*nbloc < 1 || *nbloc > MLCP_BMAX) {
+    fail("nbloc out of range");
+    return;
+  }
+  std::vector<int> ranks(*nbloc, *rk);
+  mlcp_hip_intertwine(base, coef, nbas, *ndof, *nbloc, ranks.data(), opid,
+                      mptr, Hcoef, mat, *TxD / *ndof, *nops, *msize, eigv,
+                      *npow, *nals, *update != 0, *ishift, *Eshift);
+}
+
+// alscpp_: F (rF terms) <- ALS fit of G (rG terms), reduc_ALS semantics:
+// penalty 1e-15 max(c_G), no final renormalisation.
+extern "C" void alscpp_(double* coef, double* base, int* nbas,
+                        const int* ndof, const int* rF, const int* rG,
+                        const int* nals) {
+  g_err.clear();
+  if (*nals < 1) return;
+  int D = *ndof, nF = *rF, nG = *rG;
+  if (nF < 1 || nF > MLCP_RMAX || nG < 1) {
+    fail("alscpp_: rank out of range");
+    return;
+  }
+  // the block is [F | G]; G is one pseudo-state of rank rG (not limited by
+  // MLCP_RMAX since it only appears as old columns)
+  int ranks[2] = {nF, std::min(nG, MLCP_RMAX)};
+  if (validate_dims(nbas, D, 1, ranks)) return;
+  if (setup_kernels()) return;
+  mlcp_session s;
+  s.ndof = D;
+  s.nbloc = 1;
+  s.nbas.assign(nbas, nbas + D);
+  s.ranks = {nF};
+  s.roff = {0, nF + nG};
+  s.Ntot = nF + nG;
+  s.off.assign(D + 1, 0);
+  for (int d = 0; d < D; d++) s.off[d + 1] = s.off[d] + nbas[d];
+  s.nD = s.off[D];
+  s.Rmax = nF;
+  s.nmax = *std::max_element(nbas, nbas + D);
+  if (check_lds(s.Rmax, s.nmax)) return;
+  if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess) {
+    fail("hipStreamCreate failed");
+    return;
+  }
+  int cap = s.Ntot;
+  if (s.fac.need((size_t)cap * s.nD) || s.coef.need(cap) || s.err.need(1))
+    return;
+  s.pool.ndof = D;
+  long mo = 0;
+  for (int d = 0; d < D; d++) {
+    s.pool.nbas[d] = nbas[d];
+    s.pool.moff[d] = mo;
+    mo += (long)cap * nbas[d];
+  }
+  s.pool.fac = s.fac.p;
+  s.pool.coef = s.coef.p;
+  DBuf<double> stage;
+  size_t nb = (size_t)s.Ntot * s.nD;
+  if (stage.need(nb + s.Ntot)) return;
+  double pen = 0.0;
+  for (int j = 0; j < nG; j++) pen = j ? std::max(pen, coef[nF + j]) : coef[nF];
+  pen *= 1.e-15;
+  std::vector<int> col = {0}, R = {nF};
+  std::vector<double> w(nG, 1.0), pv = {pen};
+  DBuf<int> dcol, dR;
+  if (dcol.put(col, s.st) || dR.put(R, s.st) || s.d_pen.put(pv, s.st) ||
+      s.lcW.put(w, s.st))
+    return;
+  if (hipMemcpyAsync(stage.p, base, nb * sizeof(double),
+                     hipMemcpyHostToDevice, s.st) != hipSuccess ||
+      hipMemcpyAsync(stage.p + nb, coef, s.Ntot * sizeof(double),
+                     hipMemcpyHostToDevice, s.st) != hipSuccess) {
+    fail("alscpp_: upload failed");
+    return;
+  }
+  hipLaunchKernelGGL(k_to_pool, dim3(512), dim3(256), 0, s.st, stage.p,
+                     stage.p + nb, s.pool, 0, s.Ntot, s.nD);
+  if (s.lcX.need((size_t)D * nF * nG) || s.lcG.need((size_t)D * nF * nF) ||
+      s.lcPart.need((size_t)((nG + LC_CHUNK - 1) / LC_CHUNK) * nF * s.nmax))
+    return;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s.st, s.err.p, 1L, 0);
+  LcArgs a = lc_args(&s, 1, dcol.p, dR.p, nF, nF, nG);
+  if (lc_als(&s, a, *nals, false) || check_err(&s, "alscpp_")) return;
+  std::vector<double> b(nb), c(s.Ntot);
+  if (download_region(&s, 0, b.data(), c.data())) return;
+  // only F's terms are written back (intertwining.cu:196-197)
+  for (int r = 0; r < nF; r++) {
+    coef[r] = c[r];
+    memcpy(base + (size_t)r * s.nD, b.data() + (size_t)r * s.nD,
+           s.nD * sizeof(double));
+  }
+}
+
+extern "C" void intwdevicemem_(const int* rk, const int* ndof,
+                               const int* nbloc, const int* TxD,
+                               const int* nops, const int* msize,
+                               const int* ncpu, const int* update,
+                               const int* nbas) {
+  (void)ncpu;
+  (void)msize;
+  int D = *ndof, B = *nbloc, R = *rk, T = *TxD / std::max(1, D);
+  long nD = 0, nmax = 0;
+  for (int d = 0; d < D; d++) {
+    nD += nbas[d];
+    nmax = std::max<long>(nmax, nbas[d]);
+  }
+  double MB = 1.0 / (1 << 20);
+  long N = (long)R * B;
+  double pool = (3.0 * N * (nD + 1)) * 8 * MB;
+  double terms = ((double)*nops * nmax * nmax * 8 + T * D * 4.0) * MB;
+  double opals = ((double)B * *nops * (R * R + nmax * R) +
+                  (double)B * D * R * R + 2.0 * B * R * R) * 8 * MB;
+  double gs = ((double)B * D * R * R + (B + 1.0) * R * nmax) * 8 * MB;
+  double upd = *update ? ((double)B * D * R * N + (double)B * N +
+                          (double)B * ((N + 63) / 64) * R * nmax) * 8 * MB
+                       : 0.0;
+  printf(" *** MI355X device memory requirements *** \n \n");
+  printf(" Factor pool (3 regions)  : %10.2f MB \n", pool);
+  printf(" Term list (dense ops)    : %10.2f MB \n", terms);
+  printf(" Operator-ALS caches      : %10.2f MB \n", opals);
+  printf(" Gram-Schmidt caches      : %10.2f MB \n", gs);
+  if (*update) printf(" Update caches            : %10.2f MB \n", upd);
+  printf(" TOTAL                    : %10.2f MB \n \n",
+         pool + terms + opals + gs + upd);
+}

@@ -1,0 +1,215 @@
+"""Host-side mirror of the reference's GPU wrappers (GPUIterate.f90) and of
+the solver cycle that calls them (Solver.f90), over libmlcp_hip.so.
+
+* ``gpu_intertwine``  GPU_Intertwine, GPUIterate.f90:89-187 -> intertwinecpp_
+* ``gpu_als``         GPU_ALS_AB,     GPUIterate.f90:212-285 -> alscpp_
+* ``gpu_memory``      Get_GPU_Memory, GPUIterate.f90:63-85   -> intwdevicemem_
+* ``Session``         device-resident block for repeated cycles
+* ``block_shift``     GetBlockShift, BlockPower.f90:20-45
+* ``solve``           the cycle loop of SolveHPsi, Solver.f90:389-442
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+from ._lib import check, lib
+from .hoplist import CPBlock, HopList
+
+
+def _ci(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def gpu_memory(H: HopList, rk: int, nbloc: int, update: bool = True,
+               ncpu: int = 1) -> None:
+    """Get_GPU_Memory: prints the device memory estimate."""
+    lib().intwdevicemem_(_ci(rk), _ci(H.ndof), _ci(nbloc),
+                         _ci(H.T * H.ndof), _ci(H.nops), _ci(H.msize),
+                         _ci(ncpu), _ci(1 if update else 0), H.nbas)
+
+
+def gpu_intertwine(Q: CPBlock, H: HopList, npow: int, nals: int,
+                   update: bool = True, ishift: int = 1, eshift: float = 0.0,
+                   noise: float = 1e-14, ncpu: int = 1) -> CPBlock:
+    """GPU_Intertwine: one intertwinecpp_ call on a uniform-rank block.
+
+    As in the reference, ``noise`` is added to every coefficient before the
+    call (GPUIterate.f90:107,143) and the rank of every state must equal the
+    first state's (GPUIterate.f90:152-158).  Returns the new block; its eigv
+    holds the GEP eigenvalues if ``update``."""
+    rk = int(Q.ranks[0])
+    if np.any(Q.ranks != rk):
+        raise ValueError("GPU_Intertwine(): bad rank (all states must share "
+                         "the rank of the first)")
+    out = Q.copy()
+    out.coef = out.coef + noise
+    L = lib()
+    L.intertwinecpp_(out.base, out.coef, out.nbas, _ci(rk), _ci(H.ndof),
+                     _ci(Q.nbloc), H.opid, H.mptr, H.coef, H.mat,
+                     _ci(H.T * H.ndof), _ci(H.nops), _ci(H.msize), out.eigv,
+                     _ci(npow), _ci(ncpu), _ci(nals), _ci(1 if update else 0),
+                     _ci(ishift), C.byref(C.c_double(eshift)))
+    err = L.mlcp_hip_last_error().decode()
+    if err:
+        raise RuntimeError(f"intertwinecpp_ failed: {err}")
+    return out
+
+
+def intertwine_ragged(Q: CPBlock, H: HopList, npow: int, nals: int,
+                      update: bool = True, ishift: int = 1,
+                      eshift: float = 0.0) -> CPBlock:
+    """One cycle on a block with per-state ranks (mlcp_hip_intertwine)."""
+    out = Q.copy()
+    check(lib().mlcp_hip_intertwine(out.base, out.coef, out.nbas, H.ndof,
+                                    out.nbloc, out.ranks, H.opid, H.mptr,
+                                    H.coef, H.mat, H.T, H.nops, H.msize,
+                                    out.eigv, npow, nals, int(update), ishift,
+                                    eshift), "mlcp_hip_intertwine")
+    return out
+
+
+def gpu_als(F_coef, F_base, G_coef, G_base, nbas, nals: int):
+    """GPU_ALS_AB: F (rank rF) <- ALS fit of G after nals sweeps."""
+    rF, rG = F_coef.size, G_coef.size
+    coef = np.ascontiguousarray(np.concatenate([F_coef, G_coef]))
+    base = np.ascontiguousarray(np.vstack([F_base, G_base]))
+    nbas = np.ascontiguousarray(nbas, dtype=np.int32)
+    L = lib()
+    L.alscpp_(coef, base, nbas, _ci(nbas.size), _ci(rF), _ci(rG), _ci(nals))
+    err = L.mlcp_hip_last_error().decode()
+    if err:
+        raise RuntimeError(f"alscpp_ failed: {err}")
+    return coef[:rF].copy(), base[:rF].copy()
+
+
+def gpu_gep(A: np.ndarray, B: np.ndarray):
+    """Generalized eigenproblem on the GPU (upper triangles used)."""
+    n = A.shape[0]
+    a = np.array(A.T, order="C", copy=True)
+    b = np.array(B.T, order="C", copy=True)
+    X = np.zeros((n, n))
+    w = np.zeros(n)
+    check(lib().mlcp_hip_gep(n, a, b, X, w), "mlcp_hip_gep")
+    return w, X.T.copy()
+
+
+class Session:
+    """A block and term list resident on the GPU across cycles."""
+
+    def __init__(self, Q: CPBlock, H: HopList):
+        self.H = H
+        self.Q = Q.copy()
+        h = C.c_void_p()
+        check(lib().mlcp_hip_session_create(
+            C.byref(h), self.Q.base, self.Q.coef, self.Q.nbas, H.ndof,
+            self.Q.nbloc, self.Q.ranks, H.opid, H.mptr, H.coef, H.mat, H.T,
+            H.nops, H.msize), "mlcp_hip_session_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mlcp_hip_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def cycle(self, eigv, npow, nals, update=True, ishift=1, eshift=0.0):
+        ev = np.ascontiguousarray(eigv, dtype=np.float64).copy()
+        check(lib().mlcp_hip_session_cycle(self.h, ev, npow, nals,
+                                           int(update), ishift, eshift),
+              "mlcp_hip_session_cycle")
+        return ev
+
+    def download(self) -> CPBlock:
+        out = self.Q.copy()
+        check(lib().mlcp_hip_session_download(self.h, out.base, out.coef),
+              "mlcp_hip_session_download")
+        return out
+
+    def upload(self, Q: CPBlock):
+        q = Q.copy()
+        check(lib().mlcp_hip_session_upload(self.h, q.base, q.coef),
+              "mlcp_hip_session_upload")
+
+    def phase_ms(self):
+        ms = np.zeros(5)
+        lib().mlcp_hip_session_phase_ms(self.h, ms)
+        return ms
+
+    def power(self, npow, ishift=1, eshift=0.0):
+        check(lib().mlcp_hip_session_power(self.h, npow, ishift, eshift),
+              "power")
+
+    def ortho(self, nals):
+        check(lib().mlcp_hip_session_ortho(self.h, nals), "ortho")
+
+    def prodhv(self, nals) -> CPBlock:
+        out = self.Q.copy()
+        check(lib().mlcp_hip_session_prodhv(self.h, nals, out.base, out.coef),
+              "prodhv")
+        return out
+
+    def gep(self, nals):
+        B = self.Q.nbloc
+        M, S, U, w = (np.zeros((B, B)), np.zeros((B, B)), np.zeros((B, B)),
+                      np.zeros(B))
+        check(lib().mlcp_hip_session_gep(self.h, nals, M, S, U, w), "gep")
+        # column-major device matrices -> numpy (i, j)
+        return M.T.copy(), S.T.copy(), U.T.copy(), w
+
+    def update(self, U, nals):
+        u = np.ascontiguousarray(np.asarray(U, dtype=np.float64).T)
+        check(lib().mlcp_hip_session_update(self.h, u, nals), "update")
+
+
+def block_shift(eigv: np.ndarray, bounds) -> float:
+    """GetBlockShift (BlockPower.f90:20-45)."""
+    nev = eigv.size
+    a = 2 * eigv[-1] - eigv[-2] if nev > 1 else 2 * eigv[0] - bounds[0]
+    return 0.5 * (a + bounds[1])
+
+
+@dataclass
+class SolveResult:
+    eigv: np.ndarray
+    history: List[np.ndarray] = field(default_factory=list)
+    cycles: int = 0
+    converged: bool = False
+    block: CPBlock = None
+
+
+def solve(session: Session, eigv0, bounds, ncycle, npow, nals, solvtol,
+          update=True) -> SolveResult:
+    """The cycle loop of SolveHPsi (Solver.f90:389-442) on a resident block:
+    E-shift from GetBlockShift, one intertwine cycle, relative rms delta
+    convergence test."""
+    eigv = np.asarray(eigv0, dtype=np.float64).copy()
+    res = SolveResult(eigv=eigv)
+    nev = eigv.size
+    for i in range(ncycle):
+        eshift = block_shift(eigv, bounds)
+        old = eigv.copy()
+        eigv = session.cycle(eigv, npow, nals, update, 1, eshift)
+        res.history.append(eigv.copy())
+        res.cycles = i + 1
+        delta = (old - eigv) / np.abs(eigv)
+        rms = np.sqrt(np.sum(delta ** 2)) / nev
+        if rms <= solvtol:
+            res.converged = True
+            break
+    res.eigv = eigv
+    return res

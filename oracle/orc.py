@@ -154,8 +154,8 @@ class C:
     def dsygv(A, B):
         """A, B: (n, n) with upper triangles set.  Returns (w, X)."""
         n = A.shape[0]
-        a = np.ascontiguousarray(A.T)
-        b = np.ascontiguousarray(B.T)
+        a = np.array(A.T, order="C", copy=True)
+        b = np.array(B.T, order="C", copy=True)
         w = np.zeros(n)
         _chk(lib().orc_dsygv(n, a, b, w), "dsygv")
         return w, a.T.copy()

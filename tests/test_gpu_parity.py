@@ -1,0 +1,190 @@
+"""GPU parity: every phase of the HIP path against the FP64 oracle, through
+the C-ABI, on seeded inputs small enough for the oracle and for full-tensor
+comparison.
+
+Tolerances (FP64 everywhere; the HIP path sums in a different order and
+solves the regularised normal equations by Cholesky where the reference
+uses LU, so agreement is to rounding amplified by a few ALS sweeps):
+  represented tensors   ||T_gpu - T_orc|| / ||T_orc|| <= 1e-9
+  eigenvalues           |w_gpu - w_orc| <= 1e-9 * max(1, |w|)
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tests.helpers import full_tensor, random_block, random_system
+
+pytestmark = pytest.mark.gpu
+
+TOL_T = 1e-9
+TOL_W = 1e-9
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def tensors(Q):
+    return [full_tensor(Q.nbas, *Q.state(s)) for s in range(Q.nbloc)]
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+CASES = [
+    ("uniform", [5, 6, 4], [4, 4, 4], 5),
+    ("ragged", [5, 6, 4], [3, 6, 4, 2], 6),
+    ("fourmode", [4, 3, 5, 4], [5, 3, 4], 7),
+]
+
+
+@pytest.mark.parametrize("name,nb,ranks,ncoup", CASES)
+def test_power_phase(hip, name, nb, ranks, ncoup):
+    from mlcp_amd.intertwine import Session
+    from oracle.orc import C
+    say("power", name, nb, ranks)
+    H = random_system(11, nb, ncoup=ncoup)
+    Q = random_block(12, nb, ranks)
+    with Session(Q, H) as s:
+        s.power(3, 1, 4.0)
+        g = s.download()
+    for i in range(Q.nbloc):
+        c, b = C.als_pow3(H, *Q.state(i), 3, 1, 4.0)
+        e = rel(full_tensor(Q.nbas, *g.state(i)), full_tensor(Q.nbas, c, b))
+        assert e < TOL_T, (i, e)
+
+
+@pytest.mark.parametrize("name,nb,ranks,ncoup", CASES)
+def test_ortho_phase(hip, name, nb, ranks, ncoup):
+    from mlcp_amd.intertwine import Session
+    from oracle.orc import C
+    say("ortho", name, nb, ranks)
+    H = random_system(21, nb, ncoup=ncoup)
+    Q = random_block(22, nb, ranks)
+    with Session(Q, H) as s:
+        s.ortho(4)
+        g = s.download()
+    o = C.als_ortho3(Q, 4)
+    for a, b in zip(tensors(g), tensors(o)):
+        assert rel(a, b) < TOL_T
+
+
+@pytest.mark.parametrize("name,nb,ranks,ncoup", CASES)
+def test_prodhv_phase(hip, name, nb, ranks, ncoup):
+    from mlcp_amd.intertwine import Session
+    from oracle.orc import C
+    say("prodhv", name, nb, ranks)
+    H = random_system(31, nb, ncoup=ncoup)
+    Q = random_block(32, nb, ranks)
+    with Session(Q, H) as s:
+        hq = s.prodhv(3)
+    for i in range(Q.nbloc):
+        c, b = C.prodhv_als3(H, *Q.state(i), 3)
+        e = rel(full_tensor(Q.nbas, *hq.state(i)), full_tensor(Q.nbas, c, b))
+        assert e < TOL_T, (i, e)
+
+
+@pytest.mark.parametrize("name,nb,ranks,ncoup", CASES)
+def test_gep_phase(hip, name, nb, ranks, ncoup):
+    from mlcp_amd.intertwine import Session
+    from oracle.orc import C
+    say("gep", name, nb, ranks)
+    H = random_system(41, nb, ncoup=ncoup)
+    Q = C.als_ortho3(random_block(42, nb, ranks), 6)
+    with Session(Q, H) as s:
+        M, S, U, w = s.gep(3)
+    Mo = C.getqhq3(H, Q, 3)
+    So = C.getoverlaps(Q)
+    iu = np.triu_indices(Q.nbloc)
+    assert np.max(np.abs(M[iu] - Mo[iu])) < 1e-9 * np.max(np.abs(Mo))
+    assert np.max(np.abs(S[iu] - So[iu])) < 1e-9
+    wo, Uo = C.dsygv(Mo, So)
+    assert np.max(np.abs(w - wo)) < TOL_W * max(1, np.max(np.abs(wo)))
+    assert np.max(np.abs(U - Uo)) < 1e-7
+
+
+@pytest.mark.parametrize("name,nb,ranks,ncoup", CASES)
+def test_update_phase(hip, name, nb, ranks, ncoup):
+    from mlcp_amd.intertwine import Session
+    from oracle.orc import C
+    say("update", name, nb, ranks)
+    rng = np.random.default_rng(51)
+    H = random_system(51, nb, ncoup=ncoup)
+    Q = random_block(52, nb, ranks)
+    U = rng.normal(size=(Q.nbloc, Q.nbloc))
+    with Session(Q, H) as s:
+        s.update(U, 3)
+        g = s.download()
+    for i in range(Q.nbloc):
+        c, b = C.sumlcvec3(*Q.state(i), Q, U[:, i], 3)
+        e = rel(full_tensor(Q.nbas, *g.state(i)), full_tensor(Q.nbas, c, b))
+        assert e < TOL_T, (i, e)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 33, 64, 130, 200])
+def test_gep_solver(hip, n):
+    from mlcp_amd.intertwine import gpu_gep
+    from oracle.orc import C
+    say("gep solver", n)
+    rng = np.random.default_rng(n)
+    a = rng.normal(size=(n, n))
+    A = a + a.T + np.diag(np.arange(n) * 3.0)
+    b = rng.normal(size=(n, n)) * 0.1
+    B = np.eye(n) + b @ b.T
+    w, X = gpu_gep(A, B)
+    wo, Xo = C.dsygv(A, B)
+    assert np.max(np.abs(w - wo)) < 1e-10 * max(1, np.max(np.abs(wo)))
+    # residual and B-orthonormality
+    assert np.max(np.abs(A @ X - B @ X * w)) < 1e-9 * max(1, np.max(np.abs(w)))
+    assert np.max(np.abs(X.T @ B @ X - np.eye(n))) < 1e-10
+    assert np.max(np.abs(X - Xo)) < 1e-7
+
+
+@pytest.mark.parametrize("name,nb,ranks,ncoup", CASES[:2])
+def test_full_cycle(hip, name, nb, ranks, ncoup):
+    from mlcp_amd.intertwine import intertwine_ragged
+    from oracle.orc import C
+    say("cycle", name, nb, ranks)
+    H = random_system(61, nb, ncoup=ncoup)
+    Q = random_block(62, nb, ranks)
+    g = intertwine_ragged(Q, H, 3, 4, True, 1, 5.0)
+    o = C.intertwine(H, Q, 3, 4, True, 1, 5.0)
+    assert np.max(np.abs(g.eigv - o.eigv)) < TOL_W * max(1, np.abs(o.eigv).max())
+    for a, b in zip(tensors(g), tensors(o)):
+        assert rel(a, b) < 1e-8
+
+
+def test_intertwinecpp_dropin(hip):
+    """The reference entry point, uniform rank, through GPU_Intertwine's
+    packing (noise 0 so that the oracle sees the same coefficients)."""
+    from mlcp_amd.intertwine import gpu_intertwine
+    from oracle.orc import C
+    say("intertwinecpp_")
+    nb = [5, 6, 4]
+    H = random_system(71, nb, ncoup=5)
+    Q = random_block(72, nb, [4, 4, 4, 4])
+    Q.eigv[:] = [1.0, 2.0, 3.0, 4.0]
+    g = gpu_intertwine(Q, H, 2, 3, True, 1, 6.0, noise=0.0)
+    o = C.intertwine(H, Q, 2, 3, True, 1, 6.0)
+    assert np.max(np.abs(g.eigv - o.eigv)) < TOL_W * np.abs(o.eigv).max()
+    # update = False keeps eigv and only orthogonalises
+    g2 = gpu_intertwine(Q, H, 2, 3, False, 1, 6.0, noise=0.0)
+    o2 = C.intertwine(H, Q, 2, 3, False, 1, 6.0)
+    assert np.array_equal(g2.eigv, Q.eigv)
+    for a, b in zip(tensors(g2), tensors(o2)):
+        assert rel(a, b) < 1e-8
+
+
+def test_alscpp_dropin(hip):
+    from mlcp_amd.intertwine import gpu_als
+    from oracle.orc import C
+    say("alscpp_")
+    nb = np.array([5, 6, 4], dtype=np.int32)
+    G = random_block(81, nb, [9])
+    F = random_block(82, nb, [3])
+    c, b = gpu_als(F.coef, F.base, G.coef, G.base, nb, 5)
+    co, bo = C.reduc_als(nb, G.coef, G.base, F.coef, F.base, 5)
+    assert rel(full_tensor(nb, c, b), full_tensor(nb, co, bo)) < TOL_T
