@@ -67,6 +67,14 @@ def lib():
     L.mlcp_hip_session_phase_ms.argtypes = [_v, _dp]
     L.mlcp_hip_session_destroy.restype = _i
     L.mlcp_hip_session_destroy.argtypes = [_v]
+    L.mlcp_hip_session_save.restype = _i
+    L.mlcp_hip_session_save.argtypes = [_v]
+    L.mlcp_hip_session_restore.restype = _i
+    L.mlcp_hip_session_restore.argtypes = [_v]
+    L.mlcp_hip_session_probe.restype = _i
+    L.mlcp_hip_session_probe.argtypes = [_v, _i, _i]
+    L.mlcp_hip_session_probe_read.restype = _i
+    L.mlcp_hip_session_probe_read.argtypes = [_v, _pd, C.POINTER(C.c_long)]
     L.mlcp_hip_session_power.restype = _i
     L.mlcp_hip_session_power.argtypes = [_v, _i, _i, _d]
     L.mlcp_hip_session_ortho.restype = _i

@@ -88,6 +88,8 @@ struct LcArgs {
 };
 
 #define LC_CHUNK 64
+#define GS_ICH 16                                    // columns per GS job
+#define GS_PCS ((MLCP_RMAX + GS_ICH - 1) / GS_ICH)   // partial sums per state
 
 // Gram-Schmidt (ALS_ORTHO_3) step for one target state iv.
 struct GsArgs {
@@ -98,9 +100,11 @@ struct GsArgs {
   const double* pen;     // penalty of state iv (device scalar)
   int rstr;
   int pstr;
+  int nmax;
   double* X;             // [B][ndof][rstr]  X_l(j,i) = <Q_l,j, F_i>
   double* G;             // [ndof][rstr]     own Gram
-  double* part;          // [iv+1][pstr]
+  double* part;          // [iv+1][pstr]  P_l(i,m) at i*n + m
+  double* pc;            // [iv][GS_PCS]  partial Gram-Schmidt coefficients
   double* bbs;           // [rstr] prod_{d!=k} G_d
   int* err;
 };

@@ -39,25 +39,37 @@ __device__ double wg_sum(double v, double* red) {
   return s;
 }
 
-// Cholesky factor (lower, in place, column major ld R) of an R x R SPD
-// matrix in LDS.  Returns false (on every thread) on a non-positive pivot.
-__device__ bool wg_cholesky(double* A, int R, int* flag) {
+// Solves A X = B for an SPD R x R matrix A (column major, LDS, destroyed)
+// and n right-hand sides stored as X[l*n + j] (row l of the solution is
+// X[l*n .. l*n+n)), by Gauss-Jordan elimination without pivoting (the
+// pivots are the positive Schur complements of the SPD matrix).  Two
+// workgroup barriers per column.  Thread t owns row t & 63 and every fourth
+// column from t >> 6, so no index division is needed (R <= 64, 256
+// threads).  colj (R), rowA (R), rowX (n) are LDS scratch.  Returns false on
+// every thread if a pivot is not positive.
+__device__ bool wg_gj_solve(double* A, double* X, int R, int n, double* colj,
+                            double* rowA, double* rowX, int* flag) {
+  const int i = TID & 63, g = TID >> 6;
   if (TID == 0) *flag = 0;
   __syncthreads();
   for (int j = 0; j < R; j++) {
-    if (TID == 0) {
-      double d = A[j + j * R];
-      if (!(d > 0.0)) *flag = 1;
-      A[j + j * R] = sqrt(fabs(d));
-    }
+    double piv = A[j + j * R];
+    if (!(piv > 0.0) && TID == 0) *flag = 1;
+    double rp = 1.0 / piv;
+    for (int r = TID; r < R; r += MLCP_NT) colj[r] = A[r + j * R];
+    for (int c = j + 1 + TID; c < R; c += MLCP_NT) rowA[c] = A[j + c * R] * rp;
+    for (int m = TID; m < n; m += MLCP_NT) rowX[m] = X[j * n + m] * rp;
     __syncthreads();
-    double djj = A[j + j * R];
-    for (int i = j + 1 + TID; i < R; i += MLCP_NT) A[i + j * R] /= djj;
-    __syncthreads();
-    int m = R - j - 1;
-    for (int e = TID; e < m * m; e += MLCP_NT) {
-      int i = j + 1 + e % m, c = j + 1 + e / m;
-      if (c <= i) A[i + c * R] -= A[i + j * R] * A[c + j * R];
+    if (i < R) {
+      double ci = colj[i];
+      if (i == j) {
+        for (int c = j + 1 + g; c < R; c += 4) A[j + c * R] = rowA[c];
+        for (int m = g; m < n; m += 4) X[j * n + m] = rowX[m];
+      } else {
+        for (int c = j + 1 + g; c < R; c += 4) A[i + c * R] -= ci * rowA[c];
+        double* xi = X + i * n;
+        for (int m = g; m < n; m += 4) xi[m] -= ci * rowX[m];
+      }
     }
     __syncthreads();
   }
@@ -66,28 +78,10 @@ __device__ bool wg_cholesky(double* A, int R, int* flag) {
   return ok;
 }
 
-// Solves L L^T x = b for the n right-hand sides stored as X[l*n + j]
-// (column j is one right-hand side), L from wg_cholesky.
-__device__ void wg_chol_solve(const double* L, int R, double* X, int n) {
-  for (int j = TID; j < n; j += MLCP_NT) {
-    for (int l = 0; l < R; l++) {
-      double s = X[l * n + j];
-      for (int m = 0; m < l; m++) s -= L[l + m * R] * X[m * n + j];
-      X[l * n + j] = s / L[l + l * R];
-    }
-    for (int l = R - 1; l >= 0; l--) {
-      double s = X[l * n + j];
-      for (int m = l + 1; m < R; m++) s -= L[m + l * R] * X[m * n + j];
-      X[l * n + j] = s / L[l + l * R];
-    }
-  }
-  __syncthreads();
-}
-
 // "Construct improved F" (e.g. ALSPow.f90:624-629): coef_l = ||X(l,:)||,
-// F_l^k = X(l,:) / coef_l, written to the pool.
+// F_l^k = X(l,:) / coef_l, written to the pool; X keeps the normalised rows.
 __device__ void wg_replace_mode(const Pool& p, int k, int c0, int R,
-                                const double* X, int* err) {
+                                double* X, int* err) {
   int n = p.nbas[k];
   for (int l = TID; l < R; l += MLCP_NT) {
     double s = 0.0;
@@ -97,12 +91,30 @@ __device__ void wg_replace_mode(const Pool& p, int k, int c0, int R,
     if (r != r || r == 0.0) atomicOr(err, 1);
   }
   __syncthreads();
-  for (int e = TID; e < R * n; e += MLCP_NT) {
-    int l = e / n, j = e % n;
+  for (int l = TID >> 6; l < R; l += MLCP_NT >> 6) {
     double r = p.coef[c0 + l];
-    colp(p, k, c0 + l)[j] = X[l * n + j] / r;
+    double* f = colp(p, k, c0 + l);
+    for (int j = TID & 63; j < n; j += 64) {
+      double v = X[l * n + j] / r;
+      X[l * n + j] = v;
+      f[j] = v;
+    }
   }
   __syncthreads();
+}
+
+// G(j,i) = <X_j, X_i> for the R rows of X (length n, LDS), written to
+// global G[j + i*R]; thread t owns row j = t & 63.
+__device__ void wg_gram_rows(const double* X, int R, int n, double* G) {
+  const int j = TID & 63;
+  if (j >= R) return;
+  const double* xj = X + j * n;
+  for (int i = TID >> 6; i < R; i += MLCP_NT >> 6) {
+    const double* xi = X + i * n;
+    double acc = 0.0;
+    for (int m = 0; m < n; m++) acc += xj[m] * xi[m];
+    G[j + i * R] = acc;
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -257,80 +269,120 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
 }
 
 // W_u = sum_{t: op(t,k)=u} coef_t prod_{d!=k} M_{op(t,d)}  and
-// BB = prod_{d!=k} G_d.  job = (s, tile of 256 elements of R x R).
+// BB = prod_{d!=k} G_d.  job = (s, tile of blockDim.x elements of R x R).
+// Every cached Gram of the element is staged once in LDS (Mc[op][thread])
+// so that the term loop reads LDS instead of re-reading L2.
 __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
                                                           const int2* jobs,
                                                           int k) {
+  extern __shared__ double Mc[];
   int2 jb = jobs[blockIdx.x];
+  int E = blockDim.x;
   int s = jb.x, R = a.R[s];
-  int e = jb.y * MLCP_NT + TID;
-  if (e >= R * R) return;
+  int e = jb.y * E + TID;
+  bool live = e < R * R;
   int D = a.p.ndof;
   const double* Ms = a.M + (long)s * a.nops * a.rstr + e;
+  if (live)
+    for (int op = 0; op < a.nops; op++) Mc[op * E + TID] = Ms[(long)op * a.rstr];
+  if (!live) return;
   const double* Gs = a.G + (long)s * D * a.rstr + e;
   double bb = 1.0;
   for (int m = 1; m < D; m++) bb *= Gs[(long)((k + m) % D) * a.rstr];
   a.BB[(long)s * a.rstr + e] = bb;
   int u0 = a.kslot_off[k], u1 = a.kslot_off[k + 1];
   const int* ord = a.korder + a.kord_off[k];
+  const double* mc = Mc + TID;
   for (int u = u0; u < u1; u++) {
     double acc = 0.0;
     for (int q = a.kslot_beg[u]; q < a.kslot_end[u]; q++) {
       int t = ord[q];
       const int* to = a.top + (long)t * D;
       double pr = a.tcoef[t];
-      for (int m = 1; m < D; m++) {
-        int dd = (k + m) % D;
-        pr *= Ms[(long)to[dd] * a.rstr];
-      }
+      for (int m = 1; m < D; m++) pr *= mc[to[(k + m) % D] * E];
       acc += pr;
     }
     a.W[((long)s * a.maxslots + (u - u0)) * a.rstr + e] = acc;
   }
 }
 
-// Right-hand side, regularised Cholesky solve and factor replacement for
-// mode k of every state.  One workgroup per state.
+// Right-hand side, regularised solve and factor replacement for mode k of
+// every state.  One workgroup per state.  The right-hand side
+//   bjk(l,j) = sum_u sum_i W_u(i,l) c_i Y_u(j,i)  [- E sum_i BB(i,l) c_i F_i^k(j)]
+// is a sum of small GEMMs; each slot's W_u (scaled by c) and a 32-column
+// chunk of Y_u are staged in LDS, each thread accumulating 4 rows l.
+#define OPS_JCH 32
 __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
                                                        int ishift,
                                                        double eshift) {
   extern __shared__ double sm[];
   __shared__ int flag;
+  __shared__ double colj[MLCP_RMAX], rowA[MLCP_RMAX], rowX[MLCP_NMAX];
   int s = blockIdx.x;
   int R = a.R[s], n = a.p.nbas[k];
-  double* X = sm;            // R*n, row l at X[l*n]
-  double* A = sm + R * n;    // R*R
+  double* X = sm;                    // R*n, row l at X[l*n]
+  double* Ws = sm + R * n;           // R*R, Ws[i*R + l] = c_i W(i,l)
+  double* Ys = Ws + R * R;           // OPS_JCH*R, Ys[i*OPS_JCH + jj]
   int u0 = a.kslot_off[k], u1 = a.kslot_off[k + 1];
   const double* cold = a.p.coef + a.ocol[s];
   const double* BB = a.BB + (long)s * a.rstr;
-  for (int e = TID; e < R * n; e += MLCP_NT) {
-    int l = e / n, j = e % n;
-    double acc = 0.0;
-    for (int u = u0; u < u1; u++) {
-      const double* W = a.W + ((long)s * a.maxslots + (u - u0)) * a.rstr;
-      const double* Y = a.Y + ((long)s * a.nops + a.kslot_op[u]) * a.ystr;
-      for (int i = 0; i < R; i++) acc += W[i + l * R] * cold[i] * Y[j + i * n];
+  for (int e = TID; e < R * n; e += MLCP_NT) X[e] = 0.0;
+  int nslot = u1 - u0 + (ishift != 0 ? 1 : 0);
+  int lq = (R + 3) / 4;              // groups of 4 rows
+  for (int q = 0; q < nslot; q++) {
+    bool shift = (q == u1 - u0);
+    const double* W =
+        shift ? BB : a.W + ((long)s * a.maxslots + q) * a.rstr;
+    double fac = shift ? -eshift : 1.0;
+    __syncthreads();
+    for (int e = TID; e < R * R; e += MLCP_NT) {
+      int i = e % R, l = e / R;
+      Ws[i * R + l] = fac * W[e] * cold[i];
     }
-    if (ishift != 0) {
-      double sh = 0.0;
-      for (int i = 0; i < R; i++)
-        sh += BB[i + l * R] * cold[i] * colp(a.p, k, a.ocol[s] + i)[j];
-      acc += -eshift * sh;
+    const double* Y =
+        shift ? nullptr : a.Y + ((long)s * a.nops + a.kslot_op[u0 + q]) * a.ystr;
+    for (int j0 = 0; j0 < n; j0 += OPS_JCH) {
+      int nj = min(OPS_JCH, n - j0);
+      __syncthreads();
+      for (int e = TID; e < nj * R; e += MLCP_NT) {
+        int jj = e % nj, i = e / nj;
+        Ys[i * OPS_JCH + jj] =
+            shift ? colp(a.p, k, a.ocol[s] + i)[j0 + jj] : Y[j0 + jj + i * n];
+      }
+      __syncthreads();
+      for (int w = TID; w < lq * nj; w += MLCP_NT) {
+        int jj = w % nj, l0 = (w / nj) * 4;
+        double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+        for (int i = 0; i < R; i++) {
+          double y = Ys[i * OPS_JCH + jj];
+          const double* wr = Ws + i * R + l0;
+          c0 += wr[0] * y;
+          if (l0 + 1 < R) c1 += wr[1] * y;
+          if (l0 + 2 < R) c2 += wr[2] * y;
+          if (l0 + 3 < R) c3 += wr[3] * y;
+        }
+        int j = j0 + jj;
+        X[l0 * n + j] += c0;
+        if (l0 + 1 < R) X[(l0 + 1) * n + j] += c1;
+        if (l0 + 2 < R) X[(l0 + 2) * n + j] += c2;
+        if (l0 + 3 < R) X[(l0 + 3) * n + j] += c3;
+      }
     }
-    if (k == 0 && ishift < 0) acc = -acc;
-    X[e] = acc;
   }
+  __syncthreads();
+  if (k == 0 && ishift < 0)
+    for (int e = TID; e < R * n; e += MLCP_NT) X[e] = -X[e];
   double pen = a.pen[s];
+  double* A = Ws;
   for (int e = TID; e < R * R; e += MLCP_NT) {
     int i = e % R, l = e / R;
     A[e] = BB[e] + (i == l ? pen : 0.0);
   }
   __syncthreads();
-  if (!wg_cholesky(A, R, &flag)) {
+  if (!wg_gj_solve(A, X, R, n, colj, rowA, rowX, &flag)) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
-  wg_chol_solve(A, R, X, n);
   wg_replace_mode(a.p, k, a.col[s], R, X, a.err);
 }
 
@@ -390,121 +442,124 @@ __global__ __launch_bounds__(MLCP_NT) void k_normcoef(Pool p, int c0, int R) {
 }
 
 // --------------------------------------------------------------------------
-// Gram-Schmidt step for state iv: workgroup l < iv handles previous state l,
-// workgroup iv the state's own term.  kprev = mode whose caches must be
-// refreshed first (-1: all modes).
+// Gram-Schmidt step for state iv (ALS_ORTHO_3, ALSUtils.f90:426-531).
+// Workgroup (l, ic) handles previous state l (l = iv: the state itself) and
+// the 16 columns i of chunk ic of the state being orthogonalised.  It
+// refreshes the cached cross Grams X_l^d(j,i) = <Q_l,j^d, F_i^d> of the mode
+// updated last (all modes first), forms PS = prod_{d!=k} X_l^d, the partial
+// Gram-Schmidt coefficient sum_{j,i} PS X_l^k c_lj c_i over its chunk, and
+// the coefficient-free right-hand side piece
+//   P_l(i,m) = sum_j c_lj PS(j,i) Q_l,j^k(m).
+// For l = iv the Grams are the state's own (G) and PS is the normal-equation
+// matrix prod_{d!=k} <F_j, F_i> of the solve.
 __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
                                                        int kprev) {
   extern __shared__ double sm[];
   __shared__ double red[8];
-  int l = blockIdx.x, iv = a.iv, D = a.p.ndof;
+  int iv = a.iv, D = a.p.ndof;
   int R = a.R[iv], cF = a.col[iv], n = a.p.nbas[k];
+  int nch = (R + GS_ICH - 1) / GS_ICH;
+  int l = blockIdx.x / nch, ic = blockIdx.x % nch;
+  int i0 = ic * GS_ICH, ni = min(GS_ICH, R - i0);
+  bool self = (l == iv);
+  int Rl = self ? R : a.R[l], cl = self ? cF : a.col[l];
+  double* Xl = self ? a.G : a.X + (long)l * D * a.rstr;
   const double* cf = a.p.coef + cF;
-  if (l < iv) {
-    int Rl = a.R[l], cl = a.col[l];
-    double* Xl = a.X + (long)l * D * a.rstr;
+  const double* cq = a.p.coef + cl;
+  const int j = TID & 63, g = TID >> 6;   // row j of state l, column group
+  double* PS = sm;                   // Rl*GS_ICH, PS[j + ii*Rl]
+  double* Qs = sm + Rl * GS_ICH;     // Rl*nmax, Qs[j*n + m]
+  double* Fs = Qs + Rl * a.nmax;     // GS_ICH*nmax, Fs[ii*n + m]
+  if (!self || kprev < 0) {
     int d0 = kprev < 0 ? 0 : kprev, d1 = kprev < 0 ? D : kprev + 1;
     for (int d = d0; d < d1; d++) {
       int nd = a.p.nbas[d];
-      for (int e = TID; e < Rl * R; e += MLCP_NT) {
-        int j = e % Rl, i = e / Rl;
-        const double* q = colp(a.p, d, cl + j);
-        const double* f = colp(a.p, d, cF + i);
-        double acc = 0.0;
-        for (int m = 0; m < nd; m++) acc += q[m] * f[m];
-        Xl[(long)d * a.rstr + e] = acc;
-      }
+      __syncthreads();
+      for (int r = TID >> 5; r < Rl; r += MLCP_NT >> 5)
+        for (int m = TID & 31; m < nd; m += 32)
+          Qs[r * nd + m] = colp(a.p, d, cl + r)[m];
+      for (int r = TID >> 5; r < ni; r += MLCP_NT >> 5)
+        for (int m = TID & 31; m < nd; m += 32)
+          Fs[r * nd + m] = colp(a.p, d, cF + i0 + r)[m];
+      __syncthreads();
+      if (j < Rl)
+        for (int ii = g; ii < ni; ii += 4) {
+          double acc = 0.0;
+          for (int m = 0; m < nd; m++) acc += Qs[j * nd + m] * Fs[ii * nd + m];
+          Xl[(long)d * a.rstr + j + (i0 + ii) * Rl] = acc;
+        }
     }
     __threadfence_block();
     __syncthreads();
-    double* PS = sm;  // Rl*R
-    const double* cl_ = a.p.coef + cl;
-    double acc = 0.0;
-    for (int e = TID; e < Rl * R; e += MLCP_NT) {
-      int j = e % Rl, i = e / Rl;
-      double ps = 1.0;
-      for (int m = 1; m < D; m++) ps *= Xl[(long)((k + m) % D) * a.rstr + e];
-      PS[e] = ps;
-      acc += ps * Xl[(long)k * a.rstr + e] * cl_[j] * cf[i];
-    }
-    double gcoef = -wg_sum(acc, red);
-    double* part = a.part + (long)l * a.pstr;
-    for (int e = TID; e < R * n; e += MLCP_NT) {
-      int i = e / n, m = e % n;
-      double s = 0.0;
-      for (int j = 0; j < Rl; j++)
-        s += cl_[j] * PS[j + i * Rl] * colp(a.p, k, cl + j)[m];
-      part[e] = gcoef * s;
-    }
-  } else {
-    if (kprev < 0) {
-      for (int d = 0; d < D; d++) {
-        int nd = a.p.nbas[d];
-        for (int e = TID; e < R * R; e += MLCP_NT) {
-          int j = e % R, i = e / R;
-          const double* x = colp(a.p, d, cF + j);
-          const double* y = colp(a.p, d, cF + i);
-          double acc = 0.0;
-          for (int m = 0; m < nd; m++) acc += x[m] * y[m];
-          a.G[(long)d * a.rstr + e] = acc;
-        }
-      }
-      __threadfence_block();
-      __syncthreads();
-    }
-    double* BBs = sm;
-    for (int e = TID; e < R * R; e += MLCP_NT) {
-      double ps = 1.0;
-      for (int m = 1; m < D; m++) ps *= a.G[(long)((k + m) % D) * a.rstr + e];
-      BBs[e] = ps;
-      a.bbs[e] = ps;
-    }
-    __syncthreads();
-    double* part = a.part + (long)iv * a.pstr;
-    for (int e = TID; e < R * n; e += MLCP_NT) {
-      int i = e / n, m = e % n;
-      double s = 0.0;
-      for (int j = 0; j < R; j++)
-        s += cf[j] * BBs[j + i * R] * colp(a.p, k, cF + j)[m];
-      part[e] = s;
-    }
   }
+  double acc = 0.0;
+  if (j < Rl)
+    for (int ii = g; ii < ni; ii += 4) {
+      int x = j + (i0 + ii) * Rl;
+      double ps = 1.0;
+      for (int m = 1; m < D; m++) ps *= Xl[(long)((k + m) % D) * a.rstr + x];
+      PS[j + ii * Rl] = ps;
+      if (self)
+        a.bbs[x] = ps;
+      else
+        acc += ps * Xl[(long)k * a.rstr + x] * cq[j] * cf[i0 + ii];
+    }
+  __syncthreads();
+  for (int r = TID >> 5; r < Rl; r += MLCP_NT >> 5)
+    for (int m = TID & 31; m < n; m += 32)
+      Qs[r * n + m] = cq[r] * colp(a.p, k, cl + r)[m];
+  if (!self) {
+    double tot = wg_sum(acc, red);
+    if (TID == 0) a.pc[(long)l * GS_PCS + ic] = tot;
+  }
+  __syncthreads();
+  double* P = a.part + (long)l * a.pstr;
+  for (int ii = TID >> 6; ii < ni; ii += MLCP_NT >> 6)
+    for (int m = TID & 63; m < n; m += 64) {
+      double v = 0.0;
+      for (int r = 0; r < Rl; r++) v += PS[r + ii * Rl] * Qs[r * n + m];
+      P[(i0 + ii) * n + m] = v;
+    }
 }
 
+// Solve of the Gram-Schmidt step: coefficients -sum of the partial sums,
+// bjk = sum_{l<iv} coef_l P_l + P_iv, regularised solve, factor
+// replacement, refresh of the own Gram of mode k and the renormalisation
+// c <- c / sqrt(c^T (BB o G_k) c) (ALSUtils.f90:483-531).
 __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
   extern __shared__ double sm[];
   __shared__ double red[8];
   __shared__ int flag;
+  __shared__ double colj[MLCP_RMAX], rowA[MLCP_RMAX], rowX[MLCP_NMAX];
   int iv = a.iv, R = a.R[iv], cF = a.col[iv], n = a.p.nbas[k];
-  double* X = sm;          // R*n
-  double* A = sm + R * n;  // R*R
-  for (int e = TID; e < R * n; e += MLCP_NT) {
+  int nch = (R + GS_ICH - 1) / GS_ICH;
+  double* X = sm;              // R*n
+  double* A = sm + R * n;      // R*R
+  double* cl = A + R * R;      // iv coefficients
+  for (int l = TID; l < iv; l += MLCP_NT) {
     double s = 0.0;
-    for (int l = 0; l <= iv; l++) s += a.part[(long)l * a.pstr + e];
-    X[e] = s;
-  }
-  for (int e = TID; e < R * R; e += MLCP_NT) {
-    int i = e % R, l = e / R;
-    A[e] = a.bbs[e] + (i == l ? *a.pen : 0.0);
+    for (int c = 0; c < nch; c++) s += a.pc[(long)l * GS_PCS + c];
+    cl[l] = -s;
   }
   __syncthreads();
-  if (!wg_cholesky(A, R, &flag)) {
+  for (int e = TID; e < R * n; e += MLCP_NT) {
+    double s = 0.0;
+    for (int l = 0; l < iv; l++) s += cl[l] * a.part[(long)l * a.pstr + e];
+    X[e] = s + a.part[(long)iv * a.pstr + e];
+  }
+  double pen = *a.pen;
+  for (int e = TID; e < R * R; e += MLCP_NT) {
+    int i = e % R, l = e / R;
+    A[e] = a.bbs[e] + (i == l ? pen : 0.0);
+  }
+  __syncthreads();
+  if (!wg_gj_solve(A, X, R, n, colj, rowA, rowX, &flag)) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
-  wg_chol_solve(A, R, X, n);
   wg_replace_mode(a.p, k, cF, R, X, a.err);
-  // UpdateBB_OMP(Q(iv),k,BBmem,.FALSE.) and the renormalisation
-  // (ALSUtils.f90:521-531)
   double* Gk = a.G + (long)k * a.rstr;
-  for (int e = TID; e < R * R; e += MLCP_NT) {
-    int j = e % R, i = e / R;
-    const double* x = colp(a.p, k, cF + j);
-    const double* y = colp(a.p, k, cF + i);
-    double acc = 0.0;
-    for (int m = 0; m < n; m++) acc += x[m] * y[m];
-    Gk[e] = acc;
-  }
+  wg_gram_rows(X, R, n, Gk);
   __threadfence_block();
   __syncthreads();
   const double* c = a.p.coef + cF;
@@ -599,6 +654,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
 __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
   extern __shared__ double sm[];
   __shared__ int flag;
+  __shared__ double colj[MLCP_RMAX], rowA[MLCP_RMAX], rowX[MLCP_NMAX];
   int t = blockIdx.x, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof;
   double* X = sm;
   double* A = sm + R * n;
@@ -617,21 +673,12 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
     A[e] = bb + (i == l ? pen : 0.0);
   }
   __syncthreads();
-  if (!wg_cholesky(A, R, &flag)) {
+  if (!wg_gj_solve(A, X, R, n, colj, rowA, rowX, &flag)) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
-  wg_chol_solve(A, R, X, n);
   wg_replace_mode(a.p, k, a.col[t], R, X, a.err);
-  double* Gk = a.G + ((long)t * D + k) * a.rstr;
-  for (int e = TID; e < R * R; e += MLCP_NT) {
-    int i = e % R, l = e / R;
-    const double* x = colp(a.p, k, a.col[t] + l);
-    const double* y = colp(a.p, k, a.col[t] + i);
-    double acc = 0.0;
-    for (int m = 0; m < n; m++) acc += x[m] * y[m];
-    Gk[i + l * R] = acc;
-  }
+  wg_gram_rows(X, R, n, a.G + ((long)t * D + k) * a.rstr);
 }
 
 // final renormalisation (ALSUtils.f90:1030-1039)

@@ -17,6 +17,7 @@
 #include "mlcp_kernels.hip"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -127,7 +128,8 @@ struct mlcp_session {
   std::vector<int> njobs_mode;
   int njobs_all = 0, njobs_had = 0;
   // Gram-Schmidt
-  DBuf<double> gsX, gsG, gsPart, gsBbs;
+  DBuf<double> gsX, gsG, gsPart, gsBbs, gsPc;
+  int had_tile = MLCP_NT;
   // linear-combination ALS
   DBuf<double> lcX, lcG, lcPart, lcW;
   DBuf<int2> lcJobs;
@@ -137,8 +139,21 @@ struct mlcp_session {
   int npj = 0;
   hipEvent_t ev[6] = {};
   double phase_ms[5] = {0, 0, 0, 0, 0};
+  // snapshot of the block (region Q) for restarts without host traffic
+  DBuf<double> snap;
+  // Gram-Schmidt phase captured as one HIP graph (replayed every cycle)
+  hipGraphExec_t gs_exec = nullptr;
+  int gs_nals = -1;
+  // per-kernel HIP-event probe
+  int probe_kid = 0, probe_used = 0;
+  std::vector<hipEvent_t> probe_ev;
+  double probe_ms = 0.0;
+  long probe_n = 0;
   ~mlcp_session() {
+    if (gs_exec) (void)hipGraphExecDestroy(gs_exec);
     for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : probe_ev)
       if (e) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -295,9 +310,20 @@ static int session_alloc(mlcp_session* s) {
   }
   s->njobs_all = (int)all.size();
   if (s->jobs_all.put(all, st)) return -1;
+  // Hadamard tile: elements per workgroup such that every operator's Gram
+  // of the tile fits in LDS
+  {
+    size_t per = (size_t)s->nops * sizeof(double);
+    int E = MLCP_NT;
+    while (E > 64 && per * E > g_lds_limit) E -= 64;
+    if (per * E > g_lds_limit)
+      return fail("too many operators (" + std::to_string(s->nops) +
+                  ") for the LDS of one workgroup");
+    s->had_tile = E;
+  }
   std::vector<int2> had;
   for (int i = 0; i < B; i++) {
-    int nt = (s->ranks[i] * s->ranks[i] + MLCP_NT - 1) / MLCP_NT;
+    int nt = (s->ranks[i] * s->ranks[i] + s->had_tile - 1) / s->had_tile;
     for (int t = 0; t < nt; t++) had.push_back(make_int2(i, t));
   }
   s->njobs_had = (int)had.size();
@@ -305,7 +331,8 @@ static int session_alloc(mlcp_session* s) {
   // Gram-Schmidt
   int pstr = s->Rmax * s->nmax;
   if (s->gsX.need((size_t)B * D * rstr) || s->gsG.need((size_t)D * rstr) ||
-      s->gsPart.need((size_t)(B + 1) * pstr) || s->gsBbs.need(rstr))
+      s->gsPart.need((size_t)(B + 1) * pstr) || s->gsBbs.need(rstr) ||
+      s->gsPc.need((size_t)(B + 1) * GS_PCS))
     return -1;
   // GEP
   std::vector<PairJob> pj;
@@ -328,11 +355,12 @@ static int session_alloc(mlcp_session* s) {
 }
 
 // Largest dynamic LDS request of any kernel for these sizes, checked once.
-static int check_lds(int Rmax, int nmax) {
+static int check_lds(int Rmax, int nmax, int nbloc) {
   size_t r = Rmax, n = nmax, b = sizeof(double);
   if (lds_ok(2 * n * r * b, "k_op_refresh") ||
-      lds_ok((r * n + r * r) * b, "k_op_solve/k_gs_solve/k_lc_solve") ||
-      lds_ok(r * r * b, "k_gs_terms") ||
+      lds_ok((r * n + r * r + OPS_JCH * r) * b, "k_op_solve") ||
+      lds_ok((r * n + r * r + nbloc) * b, "k_gs_solve/k_lc_solve") ||
+      lds_ok((r * GS_ICH + r * n + GS_ICH * n) * b, "k_gs_terms") ||
       lds_ok(((size_t)LC_CHUNK * r + (size_t)LC_CHUNK * n) * b, "k_lc_rhs"))
     return -1;
   return 0;
@@ -388,6 +416,40 @@ static int check_err(mlcp_session* s, const char* phase) {
 }
 
 // --------------------------------------------------------------------------
+// per-kernel probe: HIP events around every launch of one kernel
+
+enum {
+  KID_OP_HADAMARD = 1, KID_OP_SOLVE, KID_OP_REFRESH, KID_GS_TERMS,
+  KID_GS_SOLVE, KID_LC_RHS, KID_LC_SOLVE, KID_LC_REFRESH, KID_PAIR_INNER,
+  KID_GEP
+};
+
+static void pb(mlcp_session* s, int kid) {
+  if (s->probe_kid == kid && 2 * s->probe_used < (int)s->probe_ev.size())
+    (void)hipEventRecord(s->probe_ev[2 * s->probe_used], s->st);
+}
+
+static void pe(mlcp_session* s, int kid) {
+  if (s->probe_kid == kid && 2 * s->probe_used < (int)s->probe_ev.size()) {
+    (void)hipEventRecord(s->probe_ev[2 * s->probe_used + 1], s->st);
+    s->probe_used++;
+  }
+}
+
+static int probe_collect(mlcp_session* s) {
+  if (!s->probe_kid || !s->probe_used) return 0;
+  HCHK(hipStreamSynchronize(s->st));
+  for (int i = 0; i < s->probe_used; i++) {
+    float ms = 0.f;
+    HCHK(hipEventElapsedTime(&ms, s->probe_ev[2 * i], s->probe_ev[2 * i + 1]));
+    s->probe_ms += ms;
+  }
+  s->probe_n += s->probe_used;
+  s->probe_used = 0;
+  return 0;
+}
+
+// --------------------------------------------------------------------------
 // phases
 
 static size_t lds_refresh(mlcp_session* s, int d) {
@@ -397,8 +459,10 @@ static size_t lds_refresh(mlcp_session* s, int d) {
 static int refresh_all(mlcp_session* s, const OpArgs& a, int do_y) {
   size_t lds = 0;
   for (int d = 0; d < s->ndof; d++) lds = std::max(lds, lds_refresh(s, d));
+  pb(s, KID_OP_REFRESH);
   hipLaunchKernelGGL(k_op_refresh, dim3(s->njobs_all), dim3(MLCP_NT), lds,
                      s->st, a, (const int2*)s->jobs_all.p, do_y);
+  pe(s, KID_OP_REFRESH);
   HCHK(hipGetLastError());
   return 0;
 }
@@ -414,18 +478,25 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
   if (refresh_all(s, a, 1)) return -1;
   for (int it = 0; it < nitn; it++) {
     for (int k = 0; k < D; k++) {
-      hipLaunchKernelGGL(k_op_hadamard, dim3(s->njobs_had), dim3(MLCP_NT), 0,
+      pb(s, KID_OP_HADAMARD);
+      hipLaunchKernelGGL(k_op_hadamard, dim3(s->njobs_had), dim3(s->had_tile),
+                         (size_t)s->nops * s->had_tile * sizeof(double),
                          s->st, a, (const int2*)s->jobs_had.p, k);
-      size_t lds = ((size_t)s->Rmax * s->nbas[k] + (size_t)s->Rmax * s->Rmax) *
-                   sizeof(double);
+      pe(s, KID_OP_HADAMARD);
+      size_t lds = ((size_t)s->Rmax * s->nbas[k] + (size_t)s->Rmax * s->Rmax +
+                    (size_t)OPS_JCH * s->Rmax) * sizeof(double);
+      pb(s, KID_OP_SOLVE);
       hipLaunchKernelGGL(k_op_solve, dim3(s->nbloc), dim3(MLCP_NT), lds,
                          s->st, a, k, ishift, eshift);
+      pe(s, KID_OP_SOLVE);
       bool last = (it == nitn - 1 && k == D - 1);
       if (!last || renorm) {
+        pb(s, KID_OP_REFRESH);
         hipLaunchKernelGGL(k_op_refresh, dim3(s->njobs_mode[k]),
                            dim3(MLCP_NT), lds_refresh(s, k), s->st, a,
                            (const int2*)s->jobs_mode[k].p,
                            do_y_each ? 1 : 0);
+        pe(s, KID_OP_REFRESH);
       }
       if (renorm && k == D - 1)
         hipLaunchKernelGGL(k_op_renorm, dim3(s->nbloc), dim3(MLCP_NT), 0,
@@ -458,34 +529,76 @@ static GsArgs gs_args(mlcp_session* s, int iv) {
   g.pen = s->d_pen.p + iv;
   g.rstr = s->Rmax * s->Rmax;
   g.pstr = s->Rmax * s->nmax;
+  g.nmax = s->nmax;
   g.X = s->gsX.p;
   g.G = s->gsG.p;
   g.part = s->gsPart.p;
   g.bbs = s->gsBbs.p;
+  g.pc = s->gsPc.p;
   g.err = s->err.p;
   return g;
 }
 
+static int launch_ortho(mlcp_session* s, int nals);
+
+// Graph replay is the default; MLCP_HIP_GRAPH=0 launches kernel by kernel.
+static bool use_graphs() {
+  const char* e = getenv("MLCP_HIP_GRAPH");
+  return !(e && e[0] == '0');
+}
+
 static int phase_ortho(mlcp_session* s, int nals) {
   if (nals < 1) return 0;
+  bool probing = s->probe_kid == KID_GS_TERMS || s->probe_kid == KID_GS_SOLVE;
+  if (!use_graphs() || probing) return launch_ortho(s, nals);
+  if (!s->gs_exec || s->gs_nals != nals) {
+    if (s->gs_exec) {
+      HCHK(hipGraphExecDestroy(s->gs_exec));
+      s->gs_exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    int rc = launch_ortho(s, nals);
+    hipError_t e = hipStreamEndCapture(s->st, &g);
+    if (rc) return -1;
+    if (e != hipSuccess)
+      return fail(std::string("gram-schmidt capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&s->gs_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+      return fail(std::string("gram-schmidt instantiate: ") +
+                  hipGetErrorString(e));
+    s->gs_nals = nals;
+  }
+  HCHK(hipGraphLaunch(s->gs_exec, s->st));
+  return 0;
+}
+
+static int launch_ortho(mlcp_session* s, int nals) {
   int D = s->ndof;
   hipLaunchKernelGGL(k_normcoef, dim3(1), dim3(MLCP_NT), 0, s->st, s->pool,
                      s->roff[0], s->ranks[0]);
   hipLaunchKernelGGL(k_penalty, dim3((s->nbloc + 63) / 64), dim3(64), 0,
                      s->st, s->pool, s->d_colQ.p, s->d_R.p, 1.e-10,
                      s->d_pen.p, s->nbloc);
-  size_t lds_t = (size_t)s->Rmax * s->Rmax * sizeof(double);
+  size_t lds_t = ((size_t)s->Rmax * GS_ICH + (size_t)s->Rmax * s->nmax +
+                  (size_t)GS_ICH * s->nmax) * sizeof(double);
   for (int iv = 1; iv < s->nbloc; iv++) {
     GsArgs g = gs_args(s, iv);
     int kprev = -1;
+    int nch = (s->ranks[iv] + GS_ICH - 1) / GS_ICH;
     for (int it = 0; it < nals; it++)
       for (int k = 0; k < D; k++) {
-        hipLaunchKernelGGL(k_gs_terms, dim3(iv + 1), dim3(MLCP_NT), lds_t,
-                           s->st, g, k, kprev);
+        pb(s, KID_GS_TERMS);
+        hipLaunchKernelGGL(k_gs_terms, dim3((iv + 1) * nch), dim3(MLCP_NT),
+                           lds_t, s->st, g, k, kprev);
+        pe(s, KID_GS_TERMS);
         size_t lds = ((size_t)s->Rmax * s->nbas[k] +
-                      (size_t)s->Rmax * s->Rmax) * sizeof(double);
+                      (size_t)s->Rmax * s->Rmax + (size_t)iv) * sizeof(double);
+        pb(s, KID_GS_SOLVE);
         hipLaunchKernelGGL(k_gs_solve, dim3(1), dim3(MLCP_NT), lds, s->st, g,
                            k);
+        pe(s, KID_GS_SOLVE);
         kprev = k;
       }
   }
@@ -497,13 +610,17 @@ static int phase_gep(mlcp_session* s) {
   int B = s->nbloc;
   hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, s->st, s->QHQ.p,
                      2L * B * B, 0.0);
+  pb(s, KID_PAIR_INNER);
   hipLaunchKernelGGL(k_pair_inner, dim3(s->npj), dim3(MLCP_NT), 0, s->st,
                      s->pool, (const PairJob*)s->pj.p, s->QHQ.p);
+  pe(s, KID_PAIR_INNER);
   // QHQ at QHQ.p + B*B, S at QHQ.p
+  pb(s, KID_GEP);
   hipLaunchKernelGGL(k_gep, dim3(1), dim3(1024), 0, s->st,
                      (const double*)(s->QHQ.p + (size_t)B * B),
                      (const double*)s->QHQ.p, B, s->ws.p, s->U.p, s->eig.p,
                      s->err.p);
+  pe(s, KID_GEP);
   HCHK(hipGetLastError());
   return 0;
 }
@@ -539,22 +656,32 @@ static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
     for (int c = 0; c < a.nchunk; c++) jobs.push_back(make_int2(t, c));
   if (s->lcJobs.put(jobs, s->st)) return -1;
   int nj = (int)jobs.size();
-  for (int d = 0; d < D; d++)
+  for (int d = 0; d < D; d++) {
+    pb(s, KID_LC_REFRESH);
     hipLaunchKernelGGL(k_lc_refresh, dim3(nj), dim3(MLCP_NT), 0, s->st, a,
                        (const int2*)s->lcJobs.p, d);
+    pe(s, KID_LC_REFRESH);
+  }
   for (int it = 0; it < nitn; it++)
     for (int k = 0; k < D; k++) {
       size_t lds_r = ((size_t)LC_CHUNK * s->Rmax +
                       (size_t)LC_CHUNK * s->nbas[k]) * sizeof(double);
+      pb(s, KID_LC_RHS);
       hipLaunchKernelGGL(k_lc_rhs, dim3(nj), dim3(MLCP_NT), lds_r, s->st, a,
                          (const int2*)s->lcJobs.p, k);
+      pe(s, KID_LC_RHS);
       size_t lds_s = ((size_t)s->Rmax * s->nbas[k] +
                       (size_t)s->Rmax * s->Rmax) * sizeof(double);
+      pb(s, KID_LC_SOLVE);
       hipLaunchKernelGGL(k_lc_solve, dim3(B), dim3(MLCP_NT), lds_s, s->st, a,
                          k);
-      if (!(it == nitn - 1 && k == D - 1))
+      pe(s, KID_LC_SOLVE);
+      if (!(it == nitn - 1 && k == D - 1)) {
+        pb(s, KID_LC_REFRESH);
         hipLaunchKernelGGL(k_lc_refresh, dim3(nj), dim3(MLCP_NT), 0, s->st, a,
                            (const int2*)s->lcJobs.p, k);
+        pe(s, KID_LC_REFRESH);
+      }
     }
   if (final_norm)
     hipLaunchKernelGGL(k_lc_final, dim3(B), dim3(MLCP_NT), 0, s->st, a);
@@ -600,7 +727,7 @@ static int run_cycle(mlcp_session* s, double* eigv, int npow, int nals,
     if (phase_update(s, s->U.p, nals)) return -1;
     HCHK(hipEventRecord(s->ev[5], s->st));
   }
-  if (check_err(s, "intertwine cycle")) return -1;
+  if (check_err(s, "intertwine cycle") || probe_collect(s)) return -1;
   float ms;
   for (int i = 0; i < 5; i++) s->phase_ms[i] = 0.0;
   for (int i = 0; i < (update ? 5 : 2); i++) {
@@ -660,7 +787,7 @@ extern "C" int mlcp_hip_session_create(mlcp_session** out, const double* base,
     delete s;
     return fail("hipStreamCreate failed");
   }
-  if (check_lds(s->Rmax, s->nmax) ||
+  if (check_lds(s->Rmax, s->nmax, s->nbloc) ||
       build_terms(s, opid, mptr, hcoef, mat, T, nops, msize) ||
       session_alloc(s) || mlcp_hip_session_upload(s, base, coef)) {
     std::string m = g_err;
@@ -719,6 +846,61 @@ extern "C" int mlcp_hip_session_cycle(mlcp_session* s, double* eigv, int npow,
 
 extern "C" int mlcp_hip_session_phase_ms(mlcp_session* s, double* ms5) {
   for (int i = 0; i < 5; i++) ms5[i] = s->phase_ms[i];
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_save(mlcp_session* s) {
+  g_err.clear();
+  size_t n = (size_t)s->Ntot * s->nD;
+  if (s->snap.need(n + s->Ntot)) return -1;
+  for (int d = 0, o = 0; d < s->ndof; o += s->nbas[d], d++)
+    HCHK(hipMemcpyAsync(s->snap.p + (size_t)s->Ntot * o,
+                        s->pool.fac + s->pool.moff[d],
+                        sizeof(double) * s->Ntot * s->nbas[d],
+                        hipMemcpyDeviceToDevice, s->st));
+  HCHK(hipMemcpyAsync(s->snap.p + n, s->pool.coef, sizeof(double) * s->Ntot,
+                      hipMemcpyDeviceToDevice, s->st));
+  HCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_restore(mlcp_session* s) {
+  g_err.clear();
+  if (!s->snap.p) return fail("restore: no saved block");
+  size_t n = (size_t)s->Ntot * s->nD;
+  for (int d = 0, o = 0; d < s->ndof; o += s->nbas[d], d++)
+    HCHK(hipMemcpyAsync(s->pool.fac + s->pool.moff[d],
+                        s->snap.p + (size_t)s->Ntot * o,
+                        sizeof(double) * s->Ntot * s->nbas[d],
+                        hipMemcpyDeviceToDevice, s->st));
+  HCHK(hipMemcpyAsync(s->pool.coef, s->snap.p + n, sizeof(double) * s->Ntot,
+                      hipMemcpyDeviceToDevice, s->st));
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_probe(mlcp_session* s, int kernel_id,
+                                      int max_launches) {
+  g_err.clear();
+  for (auto& e : s->probe_ev)
+    if (e) (void)hipEventDestroy(e);
+  s->probe_ev.clear();
+  s->probe_kid = kernel_id;
+  s->probe_used = 0;
+  s->probe_ms = 0.0;
+  s->probe_n = 0;
+  if (kernel_id <= 0) return 0;
+  s->probe_ev.resize(2 * (size_t)std::max(1, max_launches), nullptr);
+  for (auto& e : s->probe_ev) HCHK(hipEventCreate(&e));
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_probe_read(mlcp_session* s, double* total_ms,
+                                           long* launches) {
+  if (probe_collect(s)) return -1;
+  *total_ms = s->probe_ms;
+  *launches = s->probe_n;
+  s->probe_ms = 0.0;
+  s->probe_n = 0;
   return 0;
 }
 
@@ -883,7 +1065,7 @@ extern "C" void alscpp_(double* coef, double* base, int* nbas,
   s.nD = s.off[D];
   s.Rmax = nF;
   s.nmax = *std::max_element(nbas, nbas + D);
-  if (check_lds(s.Rmax, s.nmax)) return;
+  if (check_lds(s.Rmax, s.nmax, 1)) return;
   if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess) {
     fail("hipStreamCreate failed");
     return;

@@ -145,6 +145,27 @@ class Session:
         check(lib().mlcp_hip_session_upload(self.h, q.base, q.coef),
               "mlcp_hip_session_upload")
 
+    def save(self):
+        check(lib().mlcp_hip_session_save(self.h), "session_save")
+
+    def restore(self):
+        check(lib().mlcp_hip_session_restore(self.h), "session_restore")
+
+    KERNELS = {"op_hadamard": 1, "op_solve": 2, "op_refresh": 3,
+               "gs_terms": 4, "gs_solve": 5, "lc_rhs": 6, "lc_solve": 7,
+               "lc_refresh": 8, "pair_inner": 9, "gep": 10}
+
+    def probe(self, kernel: str, max_launches: int = 16384):
+        kid = self.KERNELS[kernel] if kernel else 0
+        check(lib().mlcp_hip_session_probe(self.h, kid, max_launches),
+              "session_probe")
+
+    def probe_read(self):
+        ms, n = C.c_double(), C.c_long()
+        check(lib().mlcp_hip_session_probe_read(self.h, C.byref(ms),
+                                                C.byref(n)), "probe_read")
+        return ms.value, n.value
+
     def phase_ms(self):
         ms = np.zeros(5)
         lib().mlcp_hip_session_phase_ms(self.h, ms)

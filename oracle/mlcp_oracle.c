@@ -579,9 +579,11 @@ int orc_als_ortho3(int ndof, const int *nbas, int nbloc, const int *ranks,
 
 /* GetQHQ3 (BlockUtils.f90:309-343) with PRODHV_ALS3 (ishift=0) */
 static int getqhq3(const ham *h, cpv *q, int nbloc, int nitn, double *QHQ) {
-  int nD = q[0].nD, rc = 0;
+  int nD = q[0].nD, bad = 0;
   for (size_t x = 0; x < (size_t)nbloc * nbloc; x++) QHQ[x] = 0.0;
-  for (int i = 0; i < nbloc && !rc; i++) {
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+  for (int i = 0; i < nbloc; i++) {
+    int rc;
     int R = q[i].R;
     double *hc = (double *)malloc(sizeof(double) * R);
     double *hb = (double *)malloc(sizeof(double) * R * nD);
@@ -593,10 +595,11 @@ static int getqhq3(const ham *h, cpv *q, int nbloc, int nitn, double *QHQ) {
     if (!rc)
       for (int j = i; j < nbloc; j++)
         QHQ[i + (size_t)j * nbloc] = prodvv_ab(&q[j], &hq);
+    bad |= rc;
     free(hc);
     free(hb);
   }
-  return rc;
+  return bad;
 }
 
 int orc_getqhq3(int ndof, const int *nbas, int T, const int *opid,
@@ -615,6 +618,7 @@ int orc_getqhq3(int ndof, const int *nbas, int T, const int *opid,
 /* GetOverlaps (BlockUtils.f90:347-381) */
 static void getoverlaps(cpv *q, int nbloc, double *S) {
   for (size_t x = 0; x < (size_t)nbloc * nbloc; x++) S[x] = 0.0;
+#pragma omp parallel for schedule(dynamic)
   for (int i = 0; i < nbloc; i++)
     for (int j = i; j < nbloc; j++)
       S[i + (size_t)j * nbloc] = prodvv_ab(&q[j], &q[i]);
@@ -845,9 +849,15 @@ int orc_intertwine(int ndof, const int *nbas, int nbloc, const int *ranks,
   ham h = {ndof, T, opid, mptr, hcoef, mat};
   cpv *q = (cpv *)malloc(sizeof(cpv) * nbloc);
   int rc = make_block(q, ndof, nbas, nbloc, ranks, coef, base);
-  /* Iterate (Solver.f90:507-541): ALS_POW_alg on every state */
-  for (int s = 0; s < nbloc && !rc; s++)
-    rc = als_pow3(&h, &q[s], npow, ishift, eshift);
+  /* Iterate (Solver.f90:507-541): ALS_POW_alg on every state (states are
+   * independent: the OpenMP loop changes no result) */
+  if (!rc) {
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+    for (int s = 0; s < nbloc; s++)
+      bad |= als_pow3(&h, &q[s], npow, ishift, eshift);
+    rc = bad;
+  }
   /* ALS_ORTHO_alg (lowmem=3) */
   if (!rc) rc = als_ortho3(q, nbloc, nals);
   if (!rc && update) {
@@ -869,8 +879,11 @@ int orc_intertwine(int ndof, const int *nbas, int nbloc, const int *ranks,
       memcpy(ob, base, sizeof(double) * (size_t)tot * nD);
       cpv *qo = (cpv *)malloc(sizeof(cpv) * nbloc);
       make_block(qo, ndof, nbas, nbloc, ranks, oc, ob);
-      for (int i = 0; i < nbloc && !rc; i++)
-        rc = als_sumlcvec3(&q[i], qo, nbloc, QHQ + (size_t)i * nbloc, nals);
+      int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+      for (int i = 0; i < nbloc; i++)
+        bad |= als_sumlcvec3(&q[i], qo, nbloc, QHQ + (size_t)i * nbloc, nals);
+      rc = bad;
       free(qo);
       free(oc);
       free(ob);

@@ -1,0 +1,124 @@
+"""Writes the committed fixtures of tests/golden/.
+
+    python tests/golden/make_golden.py fixtures    # needs oracle/_ref (built
+                                                   # from the reference sources)
+    python tests/golden/make_golden.py workloads   # FP64 oracle solves
+
+fixtures:  small seeded inputs and the outputs of the compiled reference
+           routines (ALS_POW3, PRODHV_ALS3, ALS_ORTHO_3, GetQHQ3/GetOverlaps,
+           UpdateVecs2, reduc_ALS, RayleighQuotient2, one Iterate cycle),
+           saved as ref_<case>.npz.  The C oracle must reproduce them without
+           the reference present.
+workloads: the eigenvalues of the FP64 oracle after each cycle of the full
+           top-layer solve of the benchmark workloads (ncycle cycles of
+           GetBlockShift + one intertwine cycle), with a fingerprint of the
+           generated inputs; saved as workload_<name>.json.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(
+    __file__))))
+sys.path.insert(0, ROOT)
+
+from mlcp_amd import systems  # noqa: E402
+from mlcp_amd.intertwine import block_shift  # noqa: E402
+from oracle.orc import C, Ref  # noqa: E402
+from tests.helpers import random_block, random_system  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def fingerprint(H, Q):
+    """Rounding-tolerant identity of generated inputs."""
+    return {
+        "T": H.T, "nops": H.nops, "msize": H.msize,
+        "nbas": [int(x) for x in H.nbas],
+        "ranks_sum": int(Q.ranks.sum()),
+        "mat_sum": float(np.sum(H.mat)), "mat_abs": float(np.sum(np.abs(H.mat))),
+        "coef_sum": float(np.sum(H.coef)),
+        "q_base_abs": float(np.sum(np.abs(Q.base))),
+        "eig0": [float(x) for x in Q.eigv[:4]],
+    }
+
+
+def fixtures():
+    if not Ref.available():
+        raise SystemExit("oracle/_ref/ref_driver is not built")
+    cases = [("d3", [5, 6, 4], [4, 4, 4], 5), ("d4r", [4, 3, 5, 4], [5, 3, 4], 7)]
+    for name, nb, ranks, nc in cases:
+        H = random_system(501, nb, ncoup=nc)
+        Q = random_block(502, nb, ranks)
+        rng = np.random.default_rng(503)
+        U = rng.normal(size=(Q.nbloc, Q.nbloc))
+        pw = Ref.als_pow3(H, Q, 3, 1, 3.5)
+        hq = Ref.prodhv_als3(H, Q, 3)
+        gs = Ref.als_ortho3(H, Q, 4)
+        M, S = Ref.getqhq_overlaps(H, Q, 3)
+        up = Ref.updatevecs2(H, Q, U, 3)
+        rq = Ref.rayleigh(H, Q)
+        cyc = Ref.cycle(H, Q, 3, 4, 1, 5.0)
+        F = random_block(504, nb, [3])
+        G = random_block(505, nb, [7])
+        QF = random_block(506, nb, [3, 7])
+        QF.coef[:3], QF.base[:3] = F.coef, F.base
+        QF.coef[3:], QF.base[3:] = G.coef, G.base
+        rc, rb = Ref.reduc_als(H, QF, 4)
+        np.savez_compressed(
+            os.path.join(HERE, f"ref_{name}.npz"),
+            nbas=H.nbas, opid=H.opid, mptr=H.mptr, hcoef=H.coef, mat=H.mat,
+            ranks=Q.ranks, coef=Q.coef, base=Q.base, U=U,
+            pow_coef=pw.coef, pow_base=pw.base, hq_coef=hq.coef,
+            hq_base=hq.base, gs_coef=gs.coef, gs_base=gs.base, qhq=M, s=S,
+            up_coef=up.coef, up_base=up.base, rq=rq, cyc_coef=cyc.coef,
+            cyc_base=cyc.base, cyc_eigv=cyc.eigv, F_coef=F.coef,
+            F_base=F.base, G_coef=G.coef, G_base=G.base, red_coef=rc,
+            red_base=rb)
+        print("wrote", name)
+
+
+def solve_oracle(w, ncycle=None):
+    Q = w.Q0.copy()
+    eig = Q.eigv.copy()
+    hist, shifts = [], []
+    for c in range(ncycle or w.ncycle):
+        es = block_shift(eig, w.bounds)
+        t0 = time.time()
+        Q = C.intertwine(w.H, Q, w.npow, w.nals, True, 1, es)
+        eig = Q.eigv.copy()
+        hist.append([float(x) for x in eig])
+        shifts.append(float(es))
+        print(f"  {w.name} cycle {c + 1}: {time.time() - t0:.1f} s, "
+              f"E0 {eig[0]:.6f}", flush=True)
+    return hist, shifts
+
+
+def workloads(names):
+    makers = {"hh111": systems.hh111, "qff12": systems.qff12,
+              "qff24": systems.qff24}
+    for n in names:
+        w = makers[n]()
+        hist, shifts = solve_oracle(w)
+        out = {"workload": w.name, "units": w.units, "ncycle": w.ncycle,
+               "npow": w.npow, "nals": w.nals, "bounds": list(map(float, w.bounds)),
+               "fingerprint": fingerprint(w.H, w.Q0), "eshift": shifts,
+               "eigv": hist}
+        with open(os.path.join(HERE, f"workload_{n}.json"), "w") as f:
+            json.dump(out, f, indent=1)
+        print("wrote", n)
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "fixtures"
+    if what == "fixtures":
+        fixtures()
+    elif what == "workloads":
+        workloads(sys.argv[2:] or ["hh111", "qff12"])
+    else:
+        raise SystemExit(__doc__)
