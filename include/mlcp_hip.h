@@ -115,6 +115,10 @@ int mlcp_hip_session_prodhv(mlcp_session *s, int nals, double *hbase,
 int mlcp_hip_session_gep(mlcp_session *s, int nals, double *QHQ, double *S,
                          double *U, double *w);
 int mlcp_hip_session_update(mlcp_session *s, const double *U, int nals);
+/* Rayleigh quotient <Q_i, H Q_i> / <Q_i, Q_i> of every resident state with
+ * the exact term list (RayleighQuotient2, BlockUtils.f90:658-688; the
+ * GetQHQdiag step of Solver.f90:501-504). */
+int mlcp_hip_session_rayleigh(mlcp_session *s, double *rq);
 
 /* Generalized eigenproblem alone (one workgroup on the GPU): A, B upper
  * triangles (column major n x n); X eigenvectors, w ascending eigenvalues. */

@@ -85,6 +85,8 @@ def lib():
     L.mlcp_hip_session_gep.argtypes = [_v, _i, _dp, _dp, _dp, _dp]
     L.mlcp_hip_session_update.restype = _i
     L.mlcp_hip_session_update.argtypes = [_v, _dp, _i]
+    L.mlcp_hip_session_rayleigh.restype = _i
+    L.mlcp_hip_session_rayleigh.argtypes = [_v, _dp]
     L.mlcp_hip_gep.restype = _i
     L.mlcp_hip_gep.argtypes = [_i, _dp, _dp, _dp, _dp]
     _lib = L

@@ -89,6 +89,7 @@ struct LcArgs {
 
 #define LC_CHUNK 64
 #define GS_ICH 16                                    // columns per GS job
+#define GS_XCDS 8                                    // XCDs of the device
 #define GS_PCS ((MLCP_RMAX + GS_ICH - 1) / GS_ICH)   // partial sums per state
 
 // Gram-Schmidt (ALS_ORTHO_3) step for one target state iv.
@@ -106,5 +107,6 @@ struct GsArgs {
   double* part;          // [iv+1][pstr]  P_l(i,m) at i*n + m
   double* pc;            // [iv][GS_PCS]  partial Gram-Schmidt coefficients
   double* bbs;           // [rstr] prod_{d!=k} G_d
+  double* bjk;           // [pstr] right-hand side of the step
   int* err;
 };

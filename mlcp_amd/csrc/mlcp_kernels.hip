@@ -405,6 +405,36 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_renorm(OpArgs a) {
   for (int i = TID; i < R; i += MLCP_NT) c[i] *= r;
 }
 
+// Rayleigh quotients <F, H F> / <F, F> of every state from the operator
+// caches refreshed with Fold = F (RayleighQuotient2, BlockUtils.f90:658-688):
+// <F,HF> = sum_{i,l} c_i c_l sum_t coef_t prod_d M_{op(t,d)}(i,l).
+__global__ __launch_bounds__(MLCP_NT) void k_op_rq(OpArgs a, double* rq) {
+  __shared__ double red[8];
+  int s = blockIdx.x, R = a.R[s], D = a.p.ndof;
+  const double* c = a.p.coef + a.col[s];
+  const double* Ms = a.M + (long)s * a.nops * a.rstr;
+  const double* Gs = a.G + (long)s * D * a.rstr;
+  double num = 0.0, den = 0.0;
+  for (int e = TID; e < R * R; e += MLCP_NT) {
+    int i = e % R, l = e / R;
+    double cc = c[i] * c[l];
+    double h = 0.0;
+    for (int t = 0; t < a.T; t++) {
+      const int* to = a.top + (long)t * D;
+      double p = a.tcoef[t];
+      for (int d = 0; d < D; d++) p *= Ms[(long)to[d] * a.rstr + e];
+      h += p;
+    }
+    double g = 1.0;
+    for (int d = 0; d < D; d++) g *= Gs[(long)d * a.rstr + e];
+    num += cc * h;
+    den += cc * g;
+  }
+  num = wg_sum(num, red);
+  den = wg_sum(den, red);
+  if (TID == 0) rq[s] = num / den;
+}
+
 // --------------------------------------------------------------------------
 // NORMCOEF (MODVECVECML.f90:338-352) of one vector.
 __global__ __launch_bounds__(MLCP_NT) void k_normcoef(Pool p, int c0, int R) {
@@ -456,10 +486,14 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
                                                        int kprev) {
   extern __shared__ double sm[];
   __shared__ double red[8];
+  // Workgroups are dealt to the 8 XCDs round robin; only every 8th one works
+  // so that all Gram-Schmidt traffic stays in one XCD's L2.
+  if (blockIdx.x % GS_XCDS) return;
+  int job = blockIdx.x / GS_XCDS;
   int iv = a.iv, D = a.p.ndof;
   int R = a.R[iv], cF = a.col[iv], n = a.p.nbas[k];
   int nch = (R + GS_ICH - 1) / GS_ICH;
-  int l = blockIdx.x / nch, ic = blockIdx.x % nch;
+  int l = job / nch, ic = job % nch;
   int i0 = ic * GS_ICH, ni = min(GS_ICH, R - i0);
   bool self = (l == iv);
   int Rl = self ? R : a.R[l], cl = self ? cF : a.col[l];
@@ -522,31 +556,48 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
     }
 }
 
-// Solve of the Gram-Schmidt step: coefficients -sum of the partial sums,
-// bjk = sum_{l<iv} coef_l P_l + P_iv, regularised solve, factor
-// replacement, refresh of the own Gram of mode k and the renormalisation
-// c <- c / sqrt(c^T (BB o G_k) c) (ALSUtils.f90:483-531).
-__global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
-  extern __shared__ double sm[];
-  __shared__ double red[8];
-  __shared__ int flag;
-  __shared__ double colj[MLCP_RMAX], rowA[MLCP_RMAX], rowX[MLCP_NMAX];
-  int iv = a.iv, R = a.R[iv], cF = a.col[iv], n = a.p.nbas[k];
+// Right-hand side of the Gram-Schmidt step, spread over workgroups (on the
+// same XCD as k_gs_terms): coef_l = -sum of the partial sums and
+// bjk = sum_{l<iv} coef_l P_l + P_iv (previous states first, ALSUtils.f90
+// order), for a slice of GS_RED_E elements per workgroup.
+__global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k) {
+  __shared__ double cl[MLCP_BMAX];
+  if (blockIdx.x % GS_XCDS) return;
+  int job = blockIdx.x / GS_XCDS;
+  int iv = a.iv, R = a.R[iv], n = a.p.nbas[k];
   int nch = (R + GS_ICH - 1) / GS_ICH;
-  double* X = sm;              // R*n
-  double* A = sm + R * n;      // R*R
-  double* cl = A + R * R;      // iv coefficients
   for (int l = TID; l < iv; l += MLCP_NT) {
     double s = 0.0;
     for (int c = 0; c < nch; c++) s += a.pc[(long)l * GS_PCS + c];
     cl[l] = -s;
   }
   __syncthreads();
-  for (int e = TID; e < R * n; e += MLCP_NT) {
-    double s = 0.0;
-    for (int l = 0; l < iv; l++) s += cl[l] * a.part[(long)l * a.pstr + e];
-    X[e] = s + a.part[(long)iv * a.pstr + e];
+  int e = job * MLCP_NT + TID;
+  if (e >= R * n) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int l = 0;
+  for (; l + 3 < iv; l += 4) {
+    s0 += cl[l] * a.part[(long)l * a.pstr + e];
+    s1 += cl[l + 1] * a.part[(long)(l + 1) * a.pstr + e];
+    s2 += cl[l + 2] * a.part[(long)(l + 2) * a.pstr + e];
+    s3 += cl[l + 3] * a.part[(long)(l + 3) * a.pstr + e];
   }
+  for (; l < iv; l++) s0 += cl[l] * a.part[(long)l * a.pstr + e];
+  a.bjk[e] = ((s0 + s1) + (s2 + s3)) + a.part[(long)iv * a.pstr + e];
+}
+
+// Solve of the Gram-Schmidt step: regularised solve with the right-hand side
+// of k_gs_reduce, factor replacement, refresh of the own Gram of mode k and
+// the renormalisation c <- c / sqrt(c^T (BB o G_k) c) (ALSUtils.f90:483-531).
+__global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
+  extern __shared__ double sm[];
+  __shared__ double red[8];
+  __shared__ int flag;
+  __shared__ double colj[MLCP_RMAX], rowA[MLCP_RMAX], rowX[MLCP_NMAX];
+  int iv = a.iv, R = a.R[iv], cF = a.col[iv], n = a.p.nbas[k];
+  double* X = sm;              // R*n
+  double* A = sm + R * n;      // R*R
+  for (int e = TID; e < R * n; e += MLCP_NT) X[e] = a.bjk[e];
   double pen = *a.pen;
   for (int e = TID; e < R * R; e += MLCP_NT) {
     int i = e % R, l = e / R;

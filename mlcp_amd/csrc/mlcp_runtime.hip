@@ -128,7 +128,7 @@ struct mlcp_session {
   std::vector<int> njobs_mode;
   int njobs_all = 0, njobs_had = 0;
   // Gram-Schmidt
-  DBuf<double> gsX, gsG, gsPart, gsBbs, gsPc;
+  DBuf<double> gsX, gsG, gsPart, gsBbs, gsPc, gsBjk;
   int had_tile = MLCP_NT;
   // linear-combination ALS
   DBuf<double> lcX, lcG, lcPart, lcW;
@@ -332,7 +332,7 @@ static int session_alloc(mlcp_session* s) {
   int pstr = s->Rmax * s->nmax;
   if (s->gsX.need((size_t)B * D * rstr) || s->gsG.need((size_t)D * rstr) ||
       s->gsPart.need((size_t)(B + 1) * pstr) || s->gsBbs.need(rstr) ||
-      s->gsPc.need((size_t)(B + 1) * GS_PCS))
+      s->gsPc.need((size_t)(B + 1) * GS_PCS) || s->gsBjk.need(pstr))
     return -1;
   // GEP
   std::vector<PairJob> pj;
@@ -535,6 +535,7 @@ static GsArgs gs_args(mlcp_session* s, int iv) {
   g.part = s->gsPart.p;
   g.bbs = s->gsBbs.p;
   g.pc = s->gsPc.p;
+  g.bjk = s->gsBjk.p;
   g.err = s->err.p;
   return g;
 }
@@ -590,11 +591,14 @@ static int launch_ortho(mlcp_session* s, int nals) {
     for (int it = 0; it < nals; it++)
       for (int k = 0; k < D; k++) {
         pb(s, KID_GS_TERMS);
-        hipLaunchKernelGGL(k_gs_terms, dim3((iv + 1) * nch), dim3(MLCP_NT),
-                           lds_t, s->st, g, k, kprev);
+        hipLaunchKernelGGL(k_gs_terms, dim3((iv + 1) * nch * GS_XCDS),
+                           dim3(MLCP_NT), lds_t, s->st, g, k, kprev);
         pe(s, KID_GS_TERMS);
+        int nred = (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
+        hipLaunchKernelGGL(k_gs_reduce, dim3(nred * GS_XCDS), dim3(MLCP_NT), 0,
+                           s->st, g, k);
         size_t lds = ((size_t)s->Rmax * s->nbas[k] +
-                      (size_t)s->Rmax * s->Rmax + (size_t)iv) * sizeof(double);
+                      (size_t)s->Rmax * s->Rmax) * sizeof(double);
         pb(s, KID_GS_SOLVE);
         hipLaunchKernelGGL(k_gs_solve, dim3(1), dim3(MLCP_NT), lds, s->st, g,
                            k);
@@ -918,6 +922,19 @@ extern "C" int mlcp_hip_session_power(mlcp_session* s, int npow, int ishift,
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s->st, s->err.p, 1L, 0);
   if (phase_power(s, npow, ishift, eshift)) return -1;
   return check_err(s, "power");
+}
+
+extern "C" int mlcp_hip_session_rayleigh(mlcp_session* s, double* rq) {
+  g_err.clear();
+  OpArgs a = op_args(s, false);
+  if (refresh_all(s, a, 1)) return -1;
+  hipLaunchKernelGGL(k_op_rq, dim3(s->nbloc), dim3(MLCP_NT), 0, s->st, a,
+                     s->eig.p);
+  HCHK(hipGetLastError());
+  HCHK(hipMemcpyAsync(rq, s->eig.p, sizeof(double) * s->nbloc,
+                      hipMemcpyDeviceToHost, s->st));
+  HCHK(hipStreamSynchronize(s->st));
+  return 0;
 }
 
 extern "C" int mlcp_hip_session_ortho(mlcp_session* s, int nals) {

@@ -192,9 +192,57 @@ class Session:
         # column-major device matrices -> numpy (i, j)
         return M.T.copy(), S.T.copy(), U.T.copy(), w
 
+    def rayleigh(self):
+        rq = np.zeros(self.Q.nbloc)
+        check(lib().mlcp_hip_session_rayleigh(self.h, rq), "rayleigh")
+        return rq
+
     def update(self, U, nals):
         u = np.ascontiguousarray(np.asarray(U, dtype=np.float64).T)
         check(lib().mlcp_hip_session_update(self.h, u, nals), "update")
+
+
+def spectral_probes(nbas, rk: int, seed: int):
+    """The two start vectors of GetSpectralRange (Solver.f90:582-606): the
+    product of each mode's first (lowest) and last (highest) basis function,
+    base 1e-15 elsewhere, padded to rank rk with random unit-norm terms of
+    coefficient 1e-7 (the same random terms in both)."""
+    nbas = np.asarray(nbas, dtype=np.int32)
+    off = np.concatenate([[0], np.cumsum(nbas)])
+    nD = int(nbas.sum())
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pad = []
+    for _ in range(rk - 1):
+        x = rng.uniform(size=nD) - 0.5
+        for d in range(nbas.size):
+            x[off[d]:off[d + 1]] /= np.linalg.norm(x[off[d]:off[d + 1]])
+        pad.append(x)
+    out = []
+    for first in (True, False):
+        b = np.full((1, nD), 1e-15)
+        for d in range(nbas.size):
+            b[0, off[d] if first else off[d + 1] - 1] = 1.0
+        base = np.vstack([b] + pad) if pad else b
+        coef = np.array([1.0] + [1e-7] * (rk - 1))
+        out.append(CPBlock(nbas, [rk], coef, base))
+    return out
+
+
+def spectral_range(H: HopList, rk: int, npow: int = 10, ncyc: int = 5,
+                   seed: int = 1):
+    """GetSpectralRange (Solver.f90:563-654) on the GPU: ALS-guided power
+    iterations on the two probe vectors (shift 0 for the top of the spectrum,
+    the mid-range shift for the bottom), Rayleigh quotients as the bounds,
+    padded by 0.1% of the range."""
+    lo_v, hi_v = spectral_probes(H.nbas, rk, seed)
+    with Session(lo_v, H) as sl, Session(hi_v, H) as sh:
+        lo, hi = sl.rayleigh()[0], sh.rayleigh()[0]
+        for _ in range(ncyc):
+            sl.power(npow, 1, 0.5 * (lo + hi))
+            sh.power(npow, 1, 0.0)
+            lo, hi = sl.rayleigh()[0], sh.rayleigh()[0]
+    pad = 0.001 * (hi - lo)
+    return lo - pad, hi + pad
 
 
 def block_shift(eigv: np.ndarray, bounds) -> float:

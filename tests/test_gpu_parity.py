@@ -188,3 +188,33 @@ def test_alscpp_dropin(hip):
     c, b = gpu_als(F.coef, F.base, G.coef, G.base, nb, 5)
     co, bo = C.reduc_als(nb, G.coef, G.base, F.coef, F.base, 5)
     assert rel(full_tensor(nb, c, b), full_tensor(nb, co, bo)) < TOL_T
+
+
+@pytest.mark.parametrize("name,nb,ranks,ncoup", CASES)
+def test_rayleigh_quotients(hip, name, nb, ranks, ncoup):
+    from mlcp_amd.intertwine import Session
+    from oracle.orc import C
+    say("rayleigh", name, nb, ranks)
+    H = random_system(91, nb, ncoup=ncoup)
+    Q = random_block(92, nb, ranks)
+    with Session(Q, H) as s:
+        rq = s.rayleigh()
+    ro = C.rayleigh(H, Q)
+    assert np.max(np.abs(rq - ro)) < 1e-11 * np.abs(ro).max()
+
+
+def test_spectral_range_matches_oracle(hip):
+    from mlcp_amd.intertwine import spectral_probes, spectral_range
+    from oracle.orc import C
+    say("spectral range")
+    H = random_system(93, [5, 6, 4], ncoup=5)
+    lo, hi = spectral_range(H, 3, npow=4, ncyc=2, seed=5)
+    a, b = spectral_probes(H.nbas, 3, 5)
+    l, h = C.rayleigh(H, a)[0], C.rayleigh(H, b)[0]
+    for _ in range(2):
+        ca, ba = C.als_pow3(H, a.coef, a.base, 4, 1, 0.5 * (l + h))
+        cb, bb = C.als_pow3(H, b.coef, b.base, 4, 1, 0.0)
+        a.coef, a.base, b.coef, b.base = ca, ba, cb, bb
+        l, h = C.rayleigh(H, a)[0], C.rayleigh(H, b)[0]
+    pad = 0.001 * (h - l)
+    assert abs(lo - (l - pad)) < 1e-8 * abs(h) and abs(hi - (h + pad)) < 1e-8 * abs(h)
