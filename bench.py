@@ -49,6 +49,30 @@ def load_reference(name, w):
     return ref, ("ok" if ok else "fingerprint mismatch")
 
 
+def kernel_counts(w, kernel):
+    """Algorithmic (FLOPs, HBM bytes) of one launch of ``kernel``, averaged
+    over its launches in one cycle."""
+    H, Q = w.H, w.Q0
+    D, T = H.ndof, H.T
+    R = Q.ranks.astype(np.float64)
+    n = H.nbas.astype(np.float64)
+    N = float(R.sum())
+    if kernel == "gs_solve":
+        # per Gram-Schmidt step of state iv on mode k (R = R_iv, n = n_k):
+        # Gauss-Jordan on [BB + pen | bjk] (R^3 + 2 R^2 n), own Gram of mode
+        # k (2 R^2 n), normalisation (3 R n) and renormalisation (4 R^2);
+        # bytes: bjk and BB read, factors and Gram written (8 B each)
+        fl, by, cnt = 0.0, 0.0, 0
+        for iv in range(1, Q.nbloc):
+            r = R[iv]
+            for nk in n:
+                fl += r ** 3 + 4 * r * r * nk + 3 * r * nk + 4 * r * r
+                by += 8 * (2 * r * nk + 2 * r * r)
+                cnt += 1
+        return fl / cnt, by / cnt
+    return kernel_flops(w, kernel), None
+
+
 def kernel_flops(w, kernel):
     """Algorithmic FLOPs of one launch of ``kernel`` (averaged over the
     launches of one cycle where they differ)."""
@@ -153,7 +177,7 @@ def main():
     sess.save()
     probe = args.probe
     if probe == "auto":
-        probe = "lc_rhs"
+        probe = "gs_solve"
 
     phases = np.zeros(5)
 
@@ -187,14 +211,19 @@ def main():
     value = conv * args.steps / dt
 
     avg_ms = pms / max(pn, 1)
-    fl = kernel_flops(w, probe)
+    fl, by = kernel_counts(w, probe)
     achieved = fl / (avg_ms * 1e-3) / 1e12 if pn else None
-    roof = {"bound": "mfma", "kernel": probe,
-            "achieved": round(achieved, 4) if achieved else None,
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{probe}_{args.workload}_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+    roof = {"bound": "mfma", "kernel": "k_" + probe,
+            "achieved": round(achieved, 6) if achieved else None,
             "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_TFLOPS, 5) if achieved else None,
-            "traffic": None, "avg_launch_ms": round(avg_ms, 5),
-            "launches": pn, "flops_per_launch": fl}
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 7) if achieved else None,
+            "traffic": traffic, "avg_launch_ms": round(avg_ms, 5),
+            "launches": pn, "flops_per_launch": round(fl),
+            "alg_bytes_per_launch": round(by) if by else None}
 
     cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_sample_states,
                                                 w.ncycle)

@@ -40,16 +40,21 @@ __device__ double wg_sum(double v, double* red) {
 }
 
 // Solves A X = B for an SPD R x R matrix A (column major, LDS, destroyed)
-// and n right-hand sides stored as X[l*n + j] (row l of the solution is
-// X[l*n .. l*n+n)), by Gauss-Jordan elimination without pivoting (the
-// pivots are the positive Schur complements of the SPD matrix).  Two
-// workgroup barriers per column.  Thread t owns row t & 63 and every fourth
-// column from t >> 6, so no index division is needed (R <= 64, 256
-// threads).  colj (R), rowA (R), rowX (n) are LDS scratch.  Returns false on
-// every thread if a pivot is not positive.
-__device__ bool wg_gj_solve(double* A, double* X, int R, int n, double* colj,
-                            double* rowA, double* rowX, int* flag) {
+// and n right-hand sides stored as X[l*xs + j] (row l of the solution is
+// X[l*xs .. l*xs+n), xs odd so that rows fall in distinct LDS banks), by
+// Gauss-Jordan elimination without pivoting (the pivots are the positive
+// Schur complements of the SPD matrix).  Two workgroup barriers per column.
+// For the matrix part thread t owns row t & 63 and every fourth column from
+// t >> 6; for the right-hand sides the lanes run along a row, so neither
+// part needs index division or causes LDS bank conflicts (R <= 64, 256
+// threads).  sc is LDS scratch of 256 doubles.  Returns false on every
+// thread if a pivot is not positive.
+__device__ bool wg_gj_solve(double* A, double* X, int R, int n, int xs,
+                            double* sc, int* flag) {
   const int i = TID & 63, g = TID >> 6;
+  double* colj = sc;
+  double* rowA = sc + 64;
+  double* rowX = sc + 128;
   if (TID == 0) *flag = 0;
   __syncthreads();
   for (int j = 0; j < R; j++) {
@@ -58,18 +63,23 @@ __device__ bool wg_gj_solve(double* A, double* X, int R, int n, double* colj,
     double rp = 1.0 / piv;
     for (int r = TID; r < R; r += MLCP_NT) colj[r] = A[r + j * R];
     for (int c = j + 1 + TID; c < R; c += MLCP_NT) rowA[c] = A[j + c * R] * rp;
-    for (int m = TID; m < n; m += MLCP_NT) rowX[m] = X[j * n + m] * rp;
+    for (int m = TID; m < n; m += MLCP_NT) rowX[m] = X[j * xs + m] * rp;
     __syncthreads();
     if (i < R) {
       double ci = colj[i];
       if (i == j) {
         for (int c = j + 1 + g; c < R; c += 4) A[j + c * R] = rowA[c];
-        for (int m = g; m < n; m += 4) X[j * n + m] = rowX[m];
       } else {
         for (int c = j + 1 + g; c < R; c += 4) A[i + c * R] -= ci * rowA[c];
-        double* xi = X + i * n;
-        for (int m = g; m < n; m += 4) xi[m] -= ci * rowX[m];
       }
+    }
+    for (int r = g; r < R; r += 4) {
+      double cr = colj[r];
+      double* xr = X + r * xs;
+      if (r == j)
+        for (int m = i; m < n; m += 64) xr[m] = rowX[m];
+      else
+        for (int m = i; m < n; m += 64) xr[m] -= cr * rowX[m];
     }
     __syncthreads();
   }
@@ -81,11 +91,11 @@ __device__ bool wg_gj_solve(double* A, double* X, int R, int n, double* colj,
 // "Construct improved F" (e.g. ALSPow.f90:624-629): coef_l = ||X(l,:)||,
 // F_l^k = X(l,:) / coef_l, written to the pool; X keeps the normalised rows.
 __device__ void wg_replace_mode(const Pool& p, int k, int c0, int R,
-                                double* X, int* err) {
+                                double* X, int xs, int* err) {
   int n = p.nbas[k];
   for (int l = TID; l < R; l += MLCP_NT) {
     double s = 0.0;
-    for (int j = 0; j < n; j++) s += X[l * n + j] * X[l * n + j];
+    for (int j = 0; j < n; j++) s += X[l * xs + j] * X[l * xs + j];
     double r = sqrt(fabs(s));
     p.coef[c0 + l] = r;
     if (r != r || r == 0.0) atomicOr(err, 1);
@@ -95,22 +105,23 @@ __device__ void wg_replace_mode(const Pool& p, int k, int c0, int R,
     double r = p.coef[c0 + l];
     double* f = colp(p, k, c0 + l);
     for (int j = TID & 63; j < n; j += 64) {
-      double v = X[l * n + j] / r;
-      X[l * n + j] = v;
+      double v = X[l * xs + j] / r;
+      X[l * xs + j] = v;
       f[j] = v;
     }
   }
   __syncthreads();
 }
 
-// G(j,i) = <X_j, X_i> for the R rows of X (length n, LDS), written to
+// G(j,i) = <X_j, X_i> for the R rows of X (length n, stride xs, LDS), written to
 // global G[j + i*R]; thread t owns row j = t & 63.
-__device__ void wg_gram_rows(const double* X, int R, int n, double* G) {
+__device__ void wg_gram_rows(const double* X, int R, int n, int xs,
+                             double* G) {
   const int j = TID & 63;
   if (j >= R) return;
-  const double* xj = X + j * n;
+  const double* xj = X + j * xs;
   for (int i = TID >> 6; i < R; i += MLCP_NT >> 6) {
-    const double* xi = X + i * n;
+    const double* xi = X + i * xs;
     double acc = 0.0;
     for (int m = 0; m < n; m++) acc += xj[m] * xi[m];
     G[j + i * R] = acc;
@@ -317,16 +328,16 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
                                                        double eshift) {
   extern __shared__ double sm[];
   __shared__ int flag;
-  __shared__ double colj[MLCP_RMAX], rowA[MLCP_RMAX], rowX[MLCP_NMAX];
+  __shared__ double gjs[512];
   int s = blockIdx.x;
-  int R = a.R[s], n = a.p.nbas[k];
-  double* X = sm;                    // R*n, row l at X[l*n]
-  double* Ws = sm + R * n;           // R*R, Ws[i*R + l] = c_i W(i,l)
+  int R = a.R[s], n = a.p.nbas[k], xs = n | 1;
+  double* X = sm;                    // R*xs, row l at X[l*xs]
+  double* Ws = sm + R * xs;          // R*R, Ws[i*R + l] = c_i W(i,l)
   double* Ys = Ws + R * R;           // OPS_JCH*R, Ys[i*OPS_JCH + jj]
   int u0 = a.kslot_off[k], u1 = a.kslot_off[k + 1];
   const double* cold = a.p.coef + a.ocol[s];
   const double* BB = a.BB + (long)s * a.rstr;
-  for (int e = TID; e < R * n; e += MLCP_NT) X[e] = 0.0;
+  for (int e = TID; e < R * xs; e += MLCP_NT) X[e] = 0.0;
   int nslot = u1 - u0 + (ishift != 0 ? 1 : 0);
   int lq = (R + 3) / 4;              // groups of 4 rows
   for (int q = 0; q < nslot; q++) {
@@ -362,16 +373,16 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
           if (l0 + 3 < R) c3 += wr[3] * y;
         }
         int j = j0 + jj;
-        X[l0 * n + j] += c0;
-        if (l0 + 1 < R) X[(l0 + 1) * n + j] += c1;
-        if (l0 + 2 < R) X[(l0 + 2) * n + j] += c2;
-        if (l0 + 3 < R) X[(l0 + 3) * n + j] += c3;
+        X[l0 * xs + j] += c0;
+        if (l0 + 1 < R) X[(l0 + 1) * xs + j] += c1;
+        if (l0 + 2 < R) X[(l0 + 2) * xs + j] += c2;
+        if (l0 + 3 < R) X[(l0 + 3) * xs + j] += c3;
       }
     }
   }
   __syncthreads();
   if (k == 0 && ishift < 0)
-    for (int e = TID; e < R * n; e += MLCP_NT) X[e] = -X[e];
+    for (int e = TID; e < R * xs; e += MLCP_NT) X[e] = -X[e];
   double pen = a.pen[s];
   double* A = Ws;
   for (int e = TID; e < R * R; e += MLCP_NT) {
@@ -379,11 +390,11 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
     A[e] = BB[e] + (i == l ? pen : 0.0);
   }
   __syncthreads();
-  if (!wg_gj_solve(A, X, R, n, colj, rowA, rowX, &flag)) {
+  if (!wg_gj_solve(A, X, R, n, xs, gjs, &flag)) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
-  wg_replace_mode(a.p, k, a.col[s], R, X, a.err);
+  wg_replace_mode(a.p, k, a.col[s], R, X, xs, a.err);
 }
 
 // c <- c / sqrt(|<F,F>|) with <F,F> from the cached per-mode Grams
@@ -502,16 +513,17 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
   const double* cq = a.p.coef + cl;
   const int j = TID & 63, g = TID >> 6;   // row j of state l, column group
   double* PS = sm;                   // Rl*GS_ICH, PS[j + ii*Rl]
-  double* Qs = sm + Rl * GS_ICH;     // Rl*nmax, Qs[j*n + m]
-  double* Fs = Qs + Rl * a.nmax;     // GS_ICH*nmax, Fs[ii*n + m]
+  double* Qs = sm + Rl * GS_ICH;     // Rl*(nmax|1), Qs[j*stride + m]
+  double* Fs = Qs + Rl * (a.nmax | 1); // GS_ICH*nmax, Fs[ii*n + m]
   if (!self || kprev < 0) {
     int d0 = kprev < 0 ? 0 : kprev, d1 = kprev < 0 ? D : kprev + 1;
     for (int d = d0; d < d1; d++) {
       int nd = a.p.nbas[d];
       __syncthreads();
+      int qs = nd | 1;
       for (int r = TID >> 5; r < Rl; r += MLCP_NT >> 5)
         for (int m = TID & 31; m < nd; m += 32)
-          Qs[r * nd + m] = colp(a.p, d, cl + r)[m];
+          Qs[r * qs + m] = colp(a.p, d, cl + r)[m];
       for (int r = TID >> 5; r < ni; r += MLCP_NT >> 5)
         for (int m = TID & 31; m < nd; m += 32)
           Fs[r * nd + m] = colp(a.p, d, cF + i0 + r)[m];
@@ -519,7 +531,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
       if (j < Rl)
         for (int ii = g; ii < ni; ii += 4) {
           double acc = 0.0;
-          for (int m = 0; m < nd; m++) acc += Qs[j * nd + m] * Fs[ii * nd + m];
+          for (int m = 0; m < nd; m++) acc += Qs[j * qs + m] * Fs[ii * nd + m];
           Xl[(long)d * a.rstr + j + (i0 + ii) * Rl] = acc;
         }
     }
@@ -593,24 +605,24 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
   extern __shared__ double sm[];
   __shared__ double red[8];
   __shared__ int flag;
-  __shared__ double colj[MLCP_RMAX], rowA[MLCP_RMAX], rowX[MLCP_NMAX];
-  int iv = a.iv, R = a.R[iv], cF = a.col[iv], n = a.p.nbas[k];
-  double* X = sm;              // R*n
-  double* A = sm + R * n;      // R*R
-  for (int e = TID; e < R * n; e += MLCP_NT) X[e] = a.bjk[e];
+  __shared__ double gjs[512];
+  int iv = a.iv, R = a.R[iv], cF = a.col[iv], n = a.p.nbas[k], xs = n | 1;
+  double* X = sm;              // R*xs
+  double* A = sm + R * xs;     // R*R
+  for (int e = TID; e < R * n; e += MLCP_NT) X[(e / n) * xs + e % n] = a.bjk[e];
   double pen = *a.pen;
   for (int e = TID; e < R * R; e += MLCP_NT) {
     int i = e % R, l = e / R;
     A[e] = a.bbs[e] + (i == l ? pen : 0.0);
   }
   __syncthreads();
-  if (!wg_gj_solve(A, X, R, n, colj, rowA, rowX, &flag)) {
+  if (!wg_gj_solve(A, X, R, n, xs, gjs, &flag)) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
-  wg_replace_mode(a.p, k, cF, R, X, a.err);
+  wg_replace_mode(a.p, k, cF, R, X, xs, a.err);
   double* Gk = a.G + (long)k * a.rstr;
-  wg_gram_rows(X, R, n, Gk);
+  wg_gram_rows(X, R, n, xs, Gk);
   __threadfence_block();
   __syncthreads();
   const double* c = a.p.coef + cF;
@@ -705,15 +717,15 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
 __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
   extern __shared__ double sm[];
   __shared__ int flag;
-  __shared__ double colj[MLCP_RMAX], rowA[MLCP_RMAX], rowX[MLCP_NMAX];
-  int t = blockIdx.x, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof;
+  __shared__ double gjs[512];
+  int t = blockIdx.x, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof, xs = n | 1;
   double* X = sm;
-  double* A = sm + R * n;
+  double* A = sm + R * xs;
   for (int e = TID; e < R * n; e += MLCP_NT) {
     double s = 0.0;
     for (int ch = 0; ch < a.nchunk; ch++)
       s += a.part[((long)t * a.nchunk + ch) * a.pstr + e];
-    X[e] = s;
+    X[(e / n) * xs + e % n] = s;
   }
   const double* G = a.G + (long)t * D * a.rstr;
   double pen = a.pen[t];
@@ -724,12 +736,12 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
     A[e] = bb + (i == l ? pen : 0.0);
   }
   __syncthreads();
-  if (!wg_gj_solve(A, X, R, n, colj, rowA, rowX, &flag)) {
+  if (!wg_gj_solve(A, X, R, n, xs, gjs, &flag)) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
-  wg_replace_mode(a.p, k, a.col[t], R, X, a.err);
-  wg_gram_rows(X, R, n, a.G + ((long)t * D + k) * a.rstr);
+  wg_replace_mode(a.p, k, a.col[t], R, X, xs, a.err);
+  wg_gram_rows(X, R, n, xs, a.G + ((long)t * D + k) * a.rstr);
 }
 
 // final renormalisation (ALSUtils.f90:1030-1039)

@@ -80,13 +80,19 @@ static int setup_kernels() {
   HCHK(hipGetDevice(&dev));
   HCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock,
                              dev));
-  size_t lim = (size_t)v > 4096 ? (size_t)v - 4096 : (size_t)v;
   const void* ks[] = {(const void*)k_op_refresh, (const void*)k_op_solve,
                       (const void*)k_gs_terms,   (const void*)k_gs_solve,
-                      (const void*)k_lc_rhs,     (const void*)k_lc_solve};
-  for (const void* k : ks)
+                      (const void*)k_lc_rhs,     (const void*)k_lc_solve,
+                      (const void*)k_op_hadamard};
+  size_t lim = (size_t)v;
+  for (const void* k : ks) {
+    hipFuncAttributes fa;
+    HCHK(hipFuncGetAttributes(&fa, k));
+    size_t kl = (size_t)v - fa.sharedSizeBytes;
     HCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)lim));
+                             (int)kl));
+    lim = std::min(lim, kl);
+  }
   g_lds_limit = lim;
   done = true;
   return 0;
@@ -357,10 +363,12 @@ static int session_alloc(mlcp_session* s) {
 // Largest dynamic LDS request of any kernel for these sizes, checked once.
 static int check_lds(int Rmax, int nmax, int nbloc) {
   size_t r = Rmax, n = nmax, b = sizeof(double);
+  size_t n1 = n | 1;
+  (void)nbloc;
   if (lds_ok(2 * n * r * b, "k_op_refresh") ||
-      lds_ok((r * n + r * r + OPS_JCH * r) * b, "k_op_solve") ||
-      lds_ok((r * n + r * r + nbloc) * b, "k_gs_solve/k_lc_solve") ||
-      lds_ok((r * GS_ICH + r * n + GS_ICH * n) * b, "k_gs_terms") ||
+      lds_ok((r * n1 + r * r + OPS_JCH * r) * b, "k_op_solve") ||
+      lds_ok((r * n1 + r * r) * b, "k_gs_solve/k_lc_solve") ||
+      lds_ok((r * GS_ICH + r * n1 + GS_ICH * n) * b, "k_gs_terms") ||
       lds_ok(((size_t)LC_CHUNK * r + (size_t)LC_CHUNK * n) * b, "k_lc_rhs"))
     return -1;
   return 0;
@@ -483,8 +491,9 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
                          (size_t)s->nops * s->had_tile * sizeof(double),
                          s->st, a, (const int2*)s->jobs_had.p, k);
       pe(s, KID_OP_HADAMARD);
-      size_t lds = ((size_t)s->Rmax * s->nbas[k] + (size_t)s->Rmax * s->Rmax +
-                    (size_t)OPS_JCH * s->Rmax) * sizeof(double);
+      size_t lds = ((size_t)s->Rmax * (s->nbas[k] | 1) +
+                    (size_t)s->Rmax * s->Rmax + (size_t)OPS_JCH * s->Rmax) *
+                   sizeof(double);
       pb(s, KID_OP_SOLVE);
       hipLaunchKernelGGL(k_op_solve, dim3(s->nbloc), dim3(MLCP_NT), lds,
                          s->st, a, k, ishift, eshift);
@@ -582,7 +591,7 @@ static int launch_ortho(mlcp_session* s, int nals) {
   hipLaunchKernelGGL(k_penalty, dim3((s->nbloc + 63) / 64), dim3(64), 0,
                      s->st, s->pool, s->d_colQ.p, s->d_R.p, 1.e-10,
                      s->d_pen.p, s->nbloc);
-  size_t lds_t = ((size_t)s->Rmax * GS_ICH + (size_t)s->Rmax * s->nmax +
+  size_t lds_t = ((size_t)s->Rmax * GS_ICH + (size_t)s->Rmax * (s->nmax | 1) +
                   (size_t)GS_ICH * s->nmax) * sizeof(double);
   for (int iv = 1; iv < s->nbloc; iv++) {
     GsArgs g = gs_args(s, iv);
@@ -597,7 +606,7 @@ static int launch_ortho(mlcp_session* s, int nals) {
         int nred = (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
         hipLaunchKernelGGL(k_gs_reduce, dim3(nred * GS_XCDS), dim3(MLCP_NT), 0,
                            s->st, g, k);
-        size_t lds = ((size_t)s->Rmax * s->nbas[k] +
+        size_t lds = ((size_t)s->Rmax * (s->nbas[k] | 1) +
                       (size_t)s->Rmax * s->Rmax) * sizeof(double);
         pb(s, KID_GS_SOLVE);
         hipLaunchKernelGGL(k_gs_solve, dim3(1), dim3(MLCP_NT), lds, s->st, g,
@@ -674,7 +683,7 @@ static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
       hipLaunchKernelGGL(k_lc_rhs, dim3(nj), dim3(MLCP_NT), lds_r, s->st, a,
                          (const int2*)s->lcJobs.p, k);
       pe(s, KID_LC_RHS);
-      size_t lds_s = ((size_t)s->Rmax * s->nbas[k] +
+      size_t lds_s = ((size_t)s->Rmax * (s->nbas[k] | 1) +
                       (size_t)s->Rmax * s->Rmax) * sizeof(double);
       pb(s, KID_LC_SOLVE);
       hipLaunchKernelGGL(k_lc_solve, dim3(B), dim3(MLCP_NT), lds_s, s->st, a,
