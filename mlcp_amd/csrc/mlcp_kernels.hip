@@ -21,7 +21,7 @@
 // --------------------------------------------------------------------------
 // workgroup helpers
 
-__device__ double wg_sum(double v, double* red) {
+__device__ __forceinline__ double wg_sum(double v, double* red) {
   // 256-thread sum; red has >= MLCP_NT/64 slots
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   int w = TID >> 6, l = TID & 63;
@@ -41,48 +41,83 @@ __device__ double wg_sum(double v, double* red) {
 
 // Solves A X = B for an SPD R x R matrix A (column major, LDS, destroyed)
 // and n right-hand sides stored as X[l*xs + j] (row l of the solution is
-// X[l*xs .. l*xs+n), xs odd so that rows fall in distinct LDS banks), by
-// Gauss-Jordan elimination without pivoting (the pivots are the positive
-// Schur complements of the SPD matrix).  Two workgroup barriers per column.
-// For the matrix part thread t owns row t & 63 and every fourth column from
-// t >> 6; for the right-hand sides the lanes run along a row, so neither
-// part needs index division or causes LDS bank conflicts (R <= 64, 256
-// threads).  sc is LDS scratch of 256 doubles.  Returns false on every
-// thread if a pivot is not positive.
-__device__ bool wg_gj_solve(double* A, double* X, int R, int n, int xs,
+// X[l*xs .. l*xs+n), xs odd), by Gauss-Jordan elimination without pivoting
+// (the pivots are the positive Schur complements of the SPD matrix).
+//
+// The system is held in registers during the elimination: thread t (lane
+// i = t & 63, wave g = t >> 6) owns row i, with the 16 matrix columns
+// [16g, 16g+16) in a[] and the right-hand-side columns g, g+4, g+8, ... in
+// x[] (NX of them, n <= 4*NX).  The elimination loop runs over column
+// blocks jb (runtime) and the 16 columns qq of a block (unrolled), so the
+// pivot column is always the compile-time register a[qq] of wave jb.  Per
+// column the pivot column and the pivot row are published to one of two
+// alternating LDS buffers in sc (2 * (128 + 4*NX) doubles), followed by one
+// workgroup barrier; a buffer is rewritten only after the next barrier, when
+// every wave has finished reading it.  Every register slot is updated
+// without a guard (columns left of the pivot are never read again; rows and
+// columns beyond R and n hold zeros and stay zero), so the loads of a column
+// step issue together instead of one branch and one LDS round trip per slot.
+// R <= 64, n <= 4*NX, 256 threads.  Returns false on every thread if a pivot
+// is not positive.
+template <int NX>
+__device__ __forceinline__ bool wg_gj_solve(double* A, double* X, int R, int n, int xs,
                             double* sc, int* flag) {
-  const int i = TID & 63, g = TID >> 6;
-  double* colj = sc;
-  double* rowA = sc + 64;
-  double* rowX = sc + 128;
-  if (TID == 0) *flag = 0;
-  __syncthreads();
-  for (int j = 0; j < R; j++) {
-    double piv = A[j + j * R];
-    if (!(piv > 0.0) && TID == 0) *flag = 1;
-    double rp = 1.0 / piv;
-    for (int r = TID; r < R; r += MLCP_NT) colj[r] = A[r + j * R];
-    for (int c = j + 1 + TID; c < R; c += MLCP_NT) rowA[c] = A[j + c * R] * rp;
-    for (int m = TID; m < n; m += MLCP_NT) rowX[m] = X[j * xs + m] * rp;
-    __syncthreads();
-    if (i < R) {
-      double ci = colj[i];
-      if (i == j) {
-        for (int c = j + 1 + g; c < R; c += 4) A[j + c * R] = rowA[c];
-      } else {
-        for (int c = j + 1 + g; c < R; c += 4) A[i + c * R] -= ci * rowA[c];
+  const int i = TID & 63;
+  const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
+  constexpr int BUF = 128 + 4 * NX;
+  double a[16], x[NX];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    int c = 16 * g + q;
+    a[q] = (i < R && c < R) ? A[i + c * R] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < NX; q++) {
+    int c = g + 4 * q;
+    x[q] = (i < R && c < n) ? X[i * xs + c] : 0.0;
+  }
+  bool bad = false;
+  const int nb = (R + 15) >> 4;
+  for (int jb = 0; jb < nb; jb++) {
+#pragma unroll
+    for (int qq = 0; qq < 16; qq++) {
+      const int j = 16 * jb + qq;
+      if (j < R) {
+        double* colj = sc + (j & 1) * BUF;
+        double* rowA = colj + 64;
+        double* rowX = colj + 128;
+        if (g == jb) colj[i] = a[qq];
+        if (i == j) {
+#pragma unroll
+          for (int q = 0; q < 16; q++) rowA[16 * g + q] = a[q];
+#pragma unroll
+          for (int q = 0; q < NX; q++) rowX[g * NX + q] = x[q];
+        }
+        __syncthreads();
+        const double piv = colj[j];
+        bad |= !(piv > 0.0);
+        const double rp = 1.0 / piv;
+        // row j becomes rp * (row j); every other row i loses m * (row j)
+        const double s = (i == j) ? 0.0 : 1.0;
+        const double m = (i == j) ? -rp : colj[i] * rp;
+        if (g >= jb) {
+#pragma unroll
+          for (int q = 0; q < 16; q++)
+            a[q] = fma(-m, rowA[16 * g + q], a[q] * s);
+        }
+#pragma unroll
+        for (int q = 0; q < NX; q++)
+          x[q] = fma(-m, rowX[g * NX + q], x[q] * s);
       }
     }
-    for (int r = g; r < R; r += 4) {
-      double cr = colj[r];
-      double* xr = X + r * xs;
-      if (r == j)
-        for (int m = i; m < n; m += 64) xr[m] = rowX[m];
-      else
-        for (int m = i; m < n; m += 64) xr[m] -= cr * rowX[m];
-    }
-    __syncthreads();
   }
+  if (TID == 0) *flag = bad ? 1 : 0;
+#pragma unroll
+  for (int q = 0; q < NX; q++) {
+    int c = g + 4 * q;
+    if (i < R && c < n) X[i * xs + c] = x[q];
+  }
+  __syncthreads();
   bool ok = (*flag == 0);
   __syncthreads();
   return ok;
@@ -90,7 +125,7 @@ __device__ bool wg_gj_solve(double* A, double* X, int R, int n, int xs,
 
 // "Construct improved F" (e.g. ALSPow.f90:624-629): coef_l = ||X(l,:)||,
 // F_l^k = X(l,:) / coef_l, written to the pool; X keeps the normalised rows.
-__device__ void wg_replace_mode(const Pool& p, int k, int c0, int R,
+__device__ __forceinline__ void wg_replace_mode(const Pool& p, int k, int c0, int R,
                                 double* X, int xs, int* err) {
   int n = p.nbas[k];
   for (int l = TID; l < R; l += MLCP_NT) {
@@ -115,7 +150,7 @@ __device__ void wg_replace_mode(const Pool& p, int k, int c0, int R,
 
 // G(j,i) = <X_j, X_i> for the R rows of X (length n, stride xs, LDS), written to
 // global G[j + i*R]; thread t owns row j = t & 63.
-__device__ void wg_gram_rows(const double* X, int R, int n, int xs,
+__device__ __forceinline__ void wg_gram_rows(const double* X, int R, int n, int xs,
                              double* G) {
   const int j = TID & 63;
   if (j >= R) return;
@@ -323,6 +358,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
 // is a sum of small GEMMs; each slot's W_u (scaled by c) and a 32-column
 // chunk of Y_u are staged in LDS, each thread accumulating 4 rows l.
 #define OPS_JCH 32
+template <int NX>
 __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
                                                        int ishift,
                                                        double eshift) {
@@ -390,7 +426,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
     A[e] = BB[e] + (i == l ? pen : 0.0);
   }
   __syncthreads();
-  if (!wg_gj_solve(A, X, R, n, xs, gjs, &flag)) {
+  if (!wg_gj_solve<NX>(A, X, R, n, xs, gjs, &flag)) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
@@ -601,6 +637,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k) {
 // Solve of the Gram-Schmidt step: regularised solve with the right-hand side
 // of k_gs_reduce, factor replacement, refresh of the own Gram of mode k and
 // the renormalisation c <- c / sqrt(c^T (BB o G_k) c) (ALSUtils.f90:483-531).
+template <int NX>
 __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
   extern __shared__ double sm[];
   __shared__ double red[8];
@@ -616,7 +653,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
     A[e] = a.bbs[e] + (i == l ? pen : 0.0);
   }
   __syncthreads();
-  if (!wg_gj_solve(A, X, R, n, xs, gjs, &flag)) {
+  if (!wg_gj_solve<NX>(A, X, R, n, xs, gjs, &flag)) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
@@ -714,6 +751,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
   }
 }
 
+template <int NX>
 __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
   extern __shared__ double sm[];
   __shared__ int flag;
@@ -736,7 +774,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
     A[e] = bb + (i == l ? pen : 0.0);
   }
   __syncthreads();
-  if (!wg_gj_solve(A, X, R, n, xs, gjs, &flag)) {
+  if (!wg_gj_solve<NX>(A, X, R, n, xs, gjs, &flag)) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }

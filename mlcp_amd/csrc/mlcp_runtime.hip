@@ -80,10 +80,14 @@ static int setup_kernels() {
   HCHK(hipGetDevice(&dev));
   HCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock,
                              dev));
-  const void* ks[] = {(const void*)k_op_refresh, (const void*)k_op_solve,
-                      (const void*)k_gs_terms,   (const void*)k_gs_solve,
-                      (const void*)k_lc_rhs,     (const void*)k_lc_solve,
-                      (const void*)k_op_hadamard};
+#define NX_KERNELS(kern)                                                    \
+  (const void*)kern<8>, (const void*)kern<12>, (const void*)kern<16>,       \
+      (const void*)kern<24>, (const void*)kern<32>
+  const void* ks[] = {(const void*)k_op_refresh, (const void*)k_gs_terms,
+                      (const void*)k_lc_rhs,     (const void*)k_op_hadamard,
+                      NX_KERNELS(k_op_solve),    NX_KERNELS(k_gs_solve),
+                      NX_KERNELS(k_lc_solve)};
+#undef NX_KERNELS
   size_t lim = (size_t)v;
   for (const void* k : ks) {
     hipFuncAttributes fa;
@@ -426,6 +430,28 @@ static int check_err(mlcp_session* s, const char* phase) {
 // --------------------------------------------------------------------------
 // per-kernel probe: HIP events around every launch of one kernel
 
+// Launches the instantiation of a solve kernel whose register tiles hold
+// the right-hand sides of the session's largest mode (n <= 4 * NX).
+#define LAUNCH_NX(kern, grid, lds, ...)                                     \
+  do {                                                                    \
+    int nx_ = (s->nmax + 3) / 4;                                          \
+    if (nx_ <= 8)                                                         \
+      hipLaunchKernelGGL((kern<8>), grid, dim3(MLCP_NT), lds, s->st,      \
+                         __VA_ARGS__);                                    \
+    else if (nx_ <= 12)                                                   \
+      hipLaunchKernelGGL((kern<12>), grid, dim3(MLCP_NT), lds, s->st,     \
+                         __VA_ARGS__);                                    \
+    else if (nx_ <= 16)                                                   \
+      hipLaunchKernelGGL((kern<16>), grid, dim3(MLCP_NT), lds, s->st,     \
+                         __VA_ARGS__);                                    \
+    else if (nx_ <= 24)                                                   \
+      hipLaunchKernelGGL((kern<24>), grid, dim3(MLCP_NT), lds, s->st,     \
+                         __VA_ARGS__);                                    \
+    else                                                                  \
+      hipLaunchKernelGGL((kern<32>), grid, dim3(MLCP_NT), lds, s->st,     \
+                         __VA_ARGS__);                                    \
+  } while (0)
+
 enum {
   KID_OP_HADAMARD = 1, KID_OP_SOLVE, KID_OP_REFRESH, KID_GS_TERMS,
   KID_GS_SOLVE, KID_LC_RHS, KID_LC_SOLVE, KID_LC_REFRESH, KID_PAIR_INNER,
@@ -495,8 +521,7 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
                     (size_t)s->Rmax * s->Rmax + (size_t)OPS_JCH * s->Rmax) *
                    sizeof(double);
       pb(s, KID_OP_SOLVE);
-      hipLaunchKernelGGL(k_op_solve, dim3(s->nbloc), dim3(MLCP_NT), lds,
-                         s->st, a, k, ishift, eshift);
+      LAUNCH_NX(k_op_solve, dim3(s->nbloc), lds, a, k, ishift, eshift);
       pe(s, KID_OP_SOLVE);
       bool last = (it == nitn - 1 && k == D - 1);
       if (!last || renorm) {
@@ -609,8 +634,7 @@ static int launch_ortho(mlcp_session* s, int nals) {
         size_t lds = ((size_t)s->Rmax * (s->nbas[k] | 1) +
                       (size_t)s->Rmax * s->Rmax) * sizeof(double);
         pb(s, KID_GS_SOLVE);
-        hipLaunchKernelGGL(k_gs_solve, dim3(1), dim3(MLCP_NT), lds, s->st, g,
-                           k);
+        LAUNCH_NX(k_gs_solve, dim3(1), lds, g, k);
         pe(s, KID_GS_SOLVE);
         kprev = k;
       }
@@ -686,8 +710,7 @@ static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
       size_t lds_s = ((size_t)s->Rmax * (s->nbas[k] | 1) +
                       (size_t)s->Rmax * s->Rmax) * sizeof(double);
       pb(s, KID_LC_SOLVE);
-      hipLaunchKernelGGL(k_lc_solve, dim3(B), dim3(MLCP_NT), lds_s, s->st, a,
-                         k);
+      LAUNCH_NX(k_lc_solve, dim3(B), lds_s, a, k);
       pe(s, KID_LC_SOLVE);
       if (!(it == nitn - 1 && k == D - 1)) {
         pb(s, KID_LC_REFRESH);
