@@ -89,13 +89,14 @@ struct LcArgs {
 
 #define LC_CHUNK 64
 #define GS_ICH 16                                    // columns per GS job
-#define GS_XCDS 8                                    // XCDs of the device
 #define GS_PCS ((MLCP_RMAX + GS_ICH - 1) / GS_ICH)   // partial sums per state
 
 // Gram-Schmidt (ALS_ORTHO_3) step for one target state iv.
 struct GsArgs {
   Pool p;
   int iv;
+  int Riv, cFiv;         // rank and first column of state iv
+  int xst;               // workgroup stride: only blockIdx % xst == 0 works
   const int* col;        // first column per state
   const int* R;          // rank per state
   const double* pen;     // penalty of state iv (device scalar)

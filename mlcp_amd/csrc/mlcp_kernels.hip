@@ -19,6 +19,17 @@
 #define TID threadIdx.x
 
 // --------------------------------------------------------------------------
+// FP64 matrix core (v_mfma_f64_16x16x4f64 on gfx950): D = A B + C with
+//   A (16 x 4):  lane l holds A[l % 16][l / 16]
+//   B (4 x 16):  lane l holds B[l / 16][l % 16]
+//   C, D (16 x 16): lane l, element v holds D[l / 16 + 4 v][l % 16]
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d mfma_f64_4(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// --------------------------------------------------------------------------
 // workgroup helpers
 
 __device__ __forceinline__ double wg_sum(double v, double* red) {
@@ -39,86 +50,236 @@ __device__ __forceinline__ double wg_sum(double v, double* red) {
   return s;
 }
 
-// Solves A X = B for an SPD R x R matrix A (column major, LDS, destroyed)
-// and n right-hand sides stored as X[l*xs + j] (row l of the solution is
-// X[l*xs .. l*xs+n), xs odd), by Gauss-Jordan elimination without pivoting
-// (the pivots are the positive Schur complements of the SPD matrix).
-//
-// The system is held in registers during the elimination: thread t (lane
-// i = t & 63, wave g = t >> 6) owns row i, with the 16 matrix columns
-// [16g, 16g+16) in a[] and the right-hand-side columns g, g+4, g+8, ... in
-// x[] (NX of them, n <= 4*NX).  The elimination loop runs over column
-// blocks jb (runtime) and the 16 columns qq of a block (unrolled), so the
-// pivot column is always the compile-time register a[qq] of wave jb.  Per
-// column the pivot column and the pivot row are published to one of two
-// alternating LDS buffers in sc (2 * (128 + 4*NX) doubles), followed by one
-// workgroup barrier; a buffer is rewritten only after the next barrier, when
-// every wave has finished reading it.  Every register slot is updated
-// without a guard (columns left of the pivot are never read again; rows and
-// columns beyond R and n hold zeros and stay zero), so the loads of a column
-// step issue together instead of one branch and one LDS round trip per slot.
-// R <= 64, n <= 4*NX, 256 threads.  Returns false on every thread if a pivot
+// 1/x for a positive normal x: hardware reciprocal and two Newton steps
+// (within one ulp; the quotient is not rounded as IEEE division).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Inverse of a 4 x 4 SPD block from its lower triangle (P[r][c], r >= c)
+// through P = L D L^T:  P^-1 = L^-T D^-1 L^-1.  bad is set if a pivot of D
+// is not positive.
+__device__ __forceinline__ void inv4_spd(const double (&P)[4][4],
+                                         double (&Q)[4][4], bool& bad) {
+  const double d0 = P[0][0], r0 = rcp_nr(d0);
+  const double l10 = P[1][0] * r0, l20 = P[2][0] * r0, l30 = P[3][0] * r0;
+  const double s11 = P[1][1] - l10 * P[1][0], s21 = P[2][1] - l20 * P[1][0];
+  const double s31 = P[3][1] - l30 * P[1][0], s22 = P[2][2] - l20 * P[2][0];
+  const double s32 = P[3][2] - l30 * P[2][0], s33 = P[3][3] - l30 * P[3][0];
+  const double d1 = s11, r1 = rcp_nr(d1);
+  const double l21 = s21 * r1, l31 = s31 * r1;
+  const double t22 = s22 - l21 * s21, t32 = s32 - l31 * s21;
+  const double t33 = s33 - l31 * s31;
+  const double d2 = t22, r2 = rcp_nr(d2);
+  const double l32 = t32 * r2;
+  const double d3 = t33 - l32 * t32, r3 = rcp_nr(d3);
+  bad |= !(d0 > 0.0 && d1 > 0.0 && d2 > 0.0 && d3 > 0.0);
+  // unit lower L^-1
+  const double n10 = -l10, n21 = -l21, n32 = -l32;
+  const double n20 = -l20 - l21 * n10;
+  const double n31 = -l31 - l32 * n21;
+  const double n30 = -l30 - l31 * n10 - l32 * n20;
+  Q[3][3] = r3;
+  Q[3][2] = r3 * n32;
+  Q[3][1] = r3 * n31;
+  Q[3][0] = r3 * n30;
+  Q[2][2] = r2 + r3 * n32 * n32;
+  Q[2][1] = r2 * n21 + r3 * n32 * n31;
+  Q[2][0] = r2 * n20 + r3 * n32 * n30;
+  Q[1][1] = r1 + r2 * n21 * n21 + r3 * n31 * n31;
+  Q[1][0] = r1 * n10 + r2 * n21 * n20 + r3 * n31 * n30;
+  Q[0][0] = r0 + r1 * n10 * n10 + r2 * n20 * n20 + r3 * n30 * n30;
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = r + 1; c < 4; c++) Q[r][c] = Q[c][r];
+}
+
+// Tile slots per wave of gj_mfma: the [A | B] system has at most
+// 4 x (4 + NX/4) tiles of 16 x 16, dealt to the four waves round robin.
+template <int NX>
+struct GjT {
+  static constexpr int S = 4 + NX / 4;   // tiles per wave
+  static constexpr int CP = 64 + 4 * NX; // padded columns of [A | B]
+  static constexpr int BUF = 256 + 4 * CP;
+};
+// LDS scratch (doubles) of gj_mfma: two buffers of 4 pivot columns (4 x 64)
+// and 4 pivot rows (4 x CP).
+#define GJ_SC(NX) (2 * GjT<NX>::BUF)
+
+// Geometry of the tiled system: row tiles RT = ceil(R/16), column tiles
+// CT = RT + ceil(n/16) (the matrix part first); tile t = rt*CT + ct lives in
+// slot t/4 of wave t%4.
+struct GjGeo {
+  int RT, CT, nt;
+  __device__ GjGeo(int R, int n)
+      : RT((R + 15) >> 4), CT(((R + 15) >> 4) + ((n + 15) >> 4)),
+        nt(((R + 15) >> 4) * (((R + 15) >> 4) + ((n + 15) >> 4))) {}
+};
+
+// Loads the tiles of wave g: matrix entries A(i, c) (identity beyond R, so
+// that padded pivot blocks stay invertible) and right-hand sides B(i, x)
+// (zero beyond R and n).  Tile element v of lane l is row 16rt + l/16 + 4v,
+// column 16ct + l%16 (the MFMA result layout).
+template <int NX, typename FA, typename FB>
+__device__ __forceinline__ void gjm_load(v4d (&w)[GjT<NX>::S], int R, int n,
+                                         FA A, FB B) {
+  const int lane = TID & 63, g = __builtin_amdgcn_readfirstlane(TID >> 6);
+  const int lr = lane & 15, lq = lane >> 4;
+  const GjGeo geo(R, n);
+#pragma unroll
+  for (int s = 0; s < GjT<NX>::S; s++) {
+    const int t = 4 * s + g;
+    const int rt = t / geo.CT, ct = t % geo.CT;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      double val = 0.0;
+      if (t < geo.nt) {
+        const int i = 16 * rt + lq + 4 * v, c = 16 * ct + lr;
+        if (ct < geo.RT) {
+          val = (i < R && c < R) ? A(i, c) : (i == c ? 1.0 : 0.0);
+        } else {
+          const int x = c - 16 * geo.RT;
+          val = (i < R && x < n) ? B(i, x) : 0.0;
+        }
+      }
+      w[s][v] = val;
+    }
+  }
+}
+
+// Solves A X = B (A SPD, R x R, R <= 64; n <= 4*NX right-hand sides) by
+// Gauss-Jordan elimination without pivoting in blocks of four columns on
+// the FP64 matrix cores, with the system [A | B] held as 16 x 16 MFMA tiles
+// in registers (gjm_load).  Per block J = {j0..j0+3}: the owners publish
+// the pivot columns and pivot rows to one of two alternating LDS buffers in
+// sc (GJ_SC(NX) doubles), one workgroup barrier (a buffer is rewritten only
+// after the next barrier, when every wave has read it); every lane then
+// inverts the pivot block P (LDL^T) and forms its multipliers
+// M = A(:,J) P^-1 directly in the MFMA A-operand layout (lane l: row
+// 16rt + l%16, column l/16); the tiles holding rows J first get them
+// replaced by P^-1 W(J,:), then one MFMA per live tile applies
+// W <- W - M W(J,:) (rows J get multiplier zero).
+// Tiles left of the pivot block are dead and skipped.  On exit the
+// right-hand-side tiles hold X.  Returns false on every thread if a pivot
 // is not positive.
 template <int NX>
-__device__ __forceinline__ bool wg_gj_solve(double* A, double* X, int R, int n, int xs,
-                            double* sc, int* flag) {
-  const int i = TID & 63;
-  const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
-  constexpr int BUF = 128 + 4 * NX;
-  double a[16], x[NX];
+__device__ __forceinline__ bool gj_mfma(v4d (&w)[GjT<NX>::S], int R, int n,
+                                        double* sc, int* flag) {
+  constexpr int S = GjT<NX>::S, CP = GjT<NX>::CP, BUF = GjT<NX>::BUF;
+  const int lane = TID & 63, g = __builtin_amdgcn_readfirstlane(TID >> 6);
+  const int lr = lane & 15, lq = lane >> 4;
+  const GjGeo geo(R, n);
+  int srt[S], sct[S];
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    int c = 16 * g + q;
-    a[q] = (i < R && c < R) ? A[i + c * R] : 0.0;
-  }
-#pragma unroll
-  for (int q = 0; q < NX; q++) {
-    int c = g + 4 * q;
-    x[q] = (i < R && c < n) ? X[i * xs + c] : 0.0;
+  for (int s = 0; s < S; s++) {
+    const int t = 4 * s + g;
+    srt[s] = t < geo.nt ? t / geo.CT : -1;
+    sct[s] = t < geo.nt ? t % geo.CT : -1;
   }
   bool bad = false;
-  const int nb = (R + 15) >> 4;
-  for (int jb = 0; jb < nb; jb++) {
+  const v4d zero = {0.0, 0.0, 0.0, 0.0};
+  for (int jb = 0; jb < geo.RT; jb++) {
 #pragma unroll
-    for (int qq = 0; qq < 16; qq++) {
-      const int j = 16 * jb + qq;
-      if (j < R) {
-        double* colj = sc + (j & 1) * BUF;
-        double* rowA = colj + 64;
-        double* rowX = colj + 128;
-        if (g == jb) colj[i] = a[qq];
-        if (i == j) {
+    for (int u4 = 0; u4 < 4; u4++) {
+      const int j0 = 16 * jb + 4 * u4;
+      if (j0 < R) {
+        double* colB = sc + (u4 & 1) * BUF;  // [u*64 + row]
+        double* rowB = colB + 256;           // [u*CP + column]
+        const bool pcol = lr >= 4 * u4 && lr < 4 * u4 + 4;
 #pragma unroll
-          for (int q = 0; q < 16; q++) rowA[16 * g + q] = a[q];
+        for (int s = 0; s < S; s++) {
+          if (sct[s] == jb && pcol) {
 #pragma unroll
-          for (int q = 0; q < NX; q++) rowX[g * NX + q] = x[q];
+            for (int v = 0; v < 4; v++)
+              colB[(lr - 4 * u4) * 64 + 16 * srt[s] + lq + 4 * v] = w[s][v];
+          }
+          if (srt[s] == jb) rowB[lq * CP + 16 * sct[s] + lr] = w[s][u4];
         }
         __syncthreads();
-        const double piv = colj[j];
-        bad |= !(piv > 0.0);
-        const double rp = 1.0 / piv;
-        // row j becomes rp * (row j); every other row i loses m * (row j)
-        const double s = (i == j) ? 0.0 : 1.0;
-        const double m = (i == j) ? -rp : colj[i] * rp;
-        if (g >= jb) {
+        double P[4][4], Q[4][4];
 #pragma unroll
-          for (int q = 0; q < 16; q++)
-            a[q] = fma(-m, rowA[16 * g + q], a[q] * s);
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int c = 0; c <= r; c++) P[r][c] = colB[c * 64 + j0 + r];
+        inv4_spd(P, Q, bad);
+        // row l/16 of P^-1 (P^-1 is symmetric)
+        double prow[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          prow[c] = lq == 0 ? Q[0][c] : lq == 1 ? Q[1][c] : lq == 2 ? Q[2][c] : Q[3][c];
+        // multipliers -M(16rt + l%16, l/16) in the A-operand layout,
+        // M = A(:,J) P^-1, zero on the pivot rows
+        double mneg[4];
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++) {
+          double m = 0.0;
+          if (rt < geo.RT) {
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+              m = fma(prow[c], colB[c * 64 + 16 * rt + lr], m);
+          }
+          mneg[rt] = (rt == jb && pcol) ? 0.0 : -m;
         }
 #pragma unroll
-        for (int q = 0; q < NX; q++)
-          x[q] = fma(-m, rowX[g * NX + q], x[q] * s);
+        for (int s = 0; s < S; s++) {
+          const int rt = srt[s], ct = sct[s];
+          if (ct >= jb) {
+            const double* rb = rowB + 16 * ct + lr;
+            const double bro = rb[lq * CP];
+            if (rt == jb) {
+              // pivot rows: W(J, c) <- P^-1 W(J, c) (the update below
+              // leaves them, their multipliers being zero)
+              double f = 0.0;
+#pragma unroll
+              for (int c = 0; c < 4; c++) f = fma(prow[c], rb[c * CP], f);
+              w[s][u4] = f;
+            }
+            const double m = rt == 0 ? mneg[0] : rt == 1 ? mneg[1]
+                           : rt == 2 ? mneg[2] : mneg[3];
+            w[s] = mfma_f64_4(m, bro, w[s]);
+          }
+        }
       }
     }
   }
   if (TID == 0) *flag = bad ? 1 : 0;
-#pragma unroll
-  for (int q = 0; q < NX; q++) {
-    int c = g + 4 * q;
-    if (i < R && c < n) X[i * xs + c] = x[q];
-  }
   __syncthreads();
   bool ok = (*flag == 0);
+  __syncthreads();
+  return ok;
+}
+
+// Solves A X = B for an SPD R x R matrix A (column major, LDS) and n
+// right-hand sides stored as X[l*xs + j] (LDS, overwritten by the solution)
+// through gj_mfma.  sc holds GJ_SC(NX) doubles.
+template <int NX>
+__device__ __forceinline__ bool wg_gj_solve(const double* A, double* X, int R,
+                                            int n, int xs, double* sc,
+                                            int* flag) {
+  v4d w[GjT<NX>::S];
+  gjm_load<NX>(w, R, n, [&](int r, int c) { return A[r + c * R]; },
+               [&](int r, int c) { return X[r * xs + c]; });
+  __syncthreads();
+  const bool ok = gj_mfma<NX>(w, R, n, sc, flag);
+  const int lane = TID & 63, g = TID >> 6, lr = lane & 15, lq = lane >> 4;
+  const GjGeo geo(R, n);
+#pragma unroll
+  for (int s = 0; s < GjT<NX>::S; s++) {
+    const int t = 4 * s + g;
+    const int rt = t / geo.CT, ct = t % geo.CT;
+    if (t < geo.nt && ct >= geo.RT) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = 16 * rt + lq + 4 * v, x = 16 * (ct - geo.RT) + lr;
+        if (i < R && x < n) X[i * xs + x] = w[s][v];
+      }
+    }
+  }
   __syncthreads();
   return ok;
 }
@@ -364,7 +525,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
                                                        double eshift) {
   extern __shared__ double sm[];
   __shared__ int flag;
-  __shared__ double gjs[512];
+  __shared__ double gjs[GJ_SC(NX)];
   int s = blockIdx.x;
   int R = a.R[s], n = a.p.nbas[k], xs = n | 1;
   double* X = sm;                    // R*xs, row l at X[l*xs]
@@ -529,79 +690,156 @@ __global__ __launch_bounds__(MLCP_NT) void k_normcoef(Pool p, int c0, int R) {
 //   P_l(i,m) = sum_j c_lj PS(j,i) Q_l,j^k(m).
 // For l = iv the Grams are the state's own (G) and PS is the normal-equation
 // matrix prod_{d!=k} <F_j, F_i> of the solve.
+//
+// Both contractions run on the FP64 matrix cores.  Wave g owns rows
+// j in [16g, 16g+16) of state l: the cross Gram X^d (j x i, summed over the
+// basis) is one 16 x 16 MFMA tile per wave, and in the MFMA result layout
+// its element v is row j = 16g + l/16 + 4v, column i = l%16 -- exactly the A
+// operand of k-step 4g+v of the second contraction P = PS^T (c Q^k), so PS
+// never leaves the registers.  Each wave forms the part of P summed over its
+// own rows; the four parts are added in LDS (4 * NX * 256 doubles of dynamic
+// LDS, n <= 4*NX).  All global operands are loaded before the first use.
+template <int NX>
 __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
                                                        int kprev) {
+  constexpr int NT = NX / 4;   // 16-wide tiles of the mode-k basis
   extern __shared__ double sm[];
   __shared__ double red[8];
-  // Workgroups are dealt to the 8 XCDs round robin; only every 8th one works
-  // so that all Gram-Schmidt traffic stays in one XCD's L2.
-  if (blockIdx.x % GS_XCDS) return;
-  int job = blockIdx.x / GS_XCDS;
-  int iv = a.iv, D = a.p.ndof;
-  int R = a.R[iv], cF = a.col[iv], n = a.p.nbas[k];
-  int nch = (R + GS_ICH - 1) / GS_ICH;
-  int l = job / nch, ic = job % nch;
-  int i0 = ic * GS_ICH, ni = min(GS_ICH, R - i0);
-  bool self = (l == iv);
-  int Rl = self ? R : a.R[l], cl = self ? cF : a.col[l];
+  // Workgroups are dealt to the XCDs round robin; with a.xst = 8 only every
+  // 8th one works, which keeps all Gram-Schmidt traffic in one XCD's L2.
+  if (blockIdx.x % a.xst) return;
+  const int job = blockIdx.x / a.xst;
+  const int iv = a.iv, D = a.p.ndof;
+  const int R = a.Riv, cF = a.cFiv, n = a.p.nbas[k];
+  const int nch = (R + GS_ICH - 1) / GS_ICH;
+  const int l = job / nch, ic = job % nch;
+  const int i0 = ic * GS_ICH, ni = min(GS_ICH, R - i0);
+  const bool self = (l == iv);
+  const int Rl = self ? R : a.R[l], cl = self ? cF : a.col[l];
   double* Xl = self ? a.G : a.X + (long)l * D * a.rstr;
-  const double* cf = a.p.coef + cF;
-  const double* cq = a.p.coef + cl;
-  const int j = TID & 63, g = TID >> 6;   // row j of state l, column group
-  double* PS = sm;                   // Rl*GS_ICH, PS[j + ii*Rl]
-  double* Qs = sm + Rl * GS_ICH;     // Rl*(nmax|1), Qs[j*stride + m]
-  double* Fs = Qs + Rl * (a.nmax | 1); // GS_ICH*nmax, Fs[ii*n + m]
-  if (!self || kprev < 0) {
-    int d0 = kprev < 0 ? 0 : kprev, d1 = kprev < 0 ? D : kprev + 1;
-    for (int d = d0; d < d1; d++) {
-      int nd = a.p.nbas[d];
-      __syncthreads();
-      int qs = nd | 1;
-      for (int r = TID >> 5; r < Rl; r += MLCP_NT >> 5)
-        for (int m = TID & 31; m < nd; m += 32)
-          Qs[r * qs + m] = colp(a.p, d, cl + r)[m];
-      for (int r = TID >> 5; r < ni; r += MLCP_NT >> 5)
-        for (int m = TID & 31; m < nd; m += 32)
-          Fs[r * nd + m] = colp(a.p, d, cF + i0 + r)[m];
-      __syncthreads();
-      if (j < Rl)
-        for (int ii = g; ii < ni; ii += 4) {
-          double acc = 0.0;
-          for (int m = 0; m < nd; m++) acc += Qs[j * qs + m] * Fs[ii * nd + m];
-          Xl[(long)d * a.rstr + j + (i0 + ii) * Rl] = acc;
-        }
+  const int lane = TID & 63;
+  const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
+  const int lr = lane & 15, lq = lane >> 4;
+  const bool rows = 16 * g < Rl;
+  const bool icol = lr < ni;
+  // result-layout entries of this lane: (j_v = 16g + lq + 4v, i = lr)
+  bool ok[4];
+  long xo[4];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int j = 16 * g + lq + 4 * v;
+    ok[v] = icol && j < Rl;
+    xo[v] = ok[v] ? j + (long)(i0 + lr) * Rl : 0;
+  }
+  double cq[4] = {0.0, 0.0, 0.0, 0.0}, qk[4][NT];
+  const double cfi = icol ? a.p.coef[cF + i0 + lr] : 0.0;
+  double ps[4] = {1.0, 1.0, 1.0, 1.0}, xk[4] = {0.0, 0.0, 0.0, 0.0};
+  if (rows) {
+    // operands of the second contraction: B(j, m) = Q_l,j^k(m), j = 4ks+lq
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int j = 16 * g + 4 * v + lq;
+      const bool jr = j < Rl;
+      cq[v] = jr ? a.p.coef[cl + j] : 0.0;
+      const double* q = colp(a.p, k, cl + (jr ? j : 0));
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const int m = 16 * t + lr;
+        qk[v][t] = (jr && m < n) ? q[m] : 0.0;
+      }
     }
-    __threadfence_block();
+    // cached cross Grams of the modes not refreshed here, four modes a round
+    for (int d0 = 0; d0 < D; d0 += 4) {
+      double xv[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int d = min(d0 + u, D - 1);
+#pragma unroll
+        for (int v = 0; v < 4; v++) xv[u][v] = Xl[(long)d * a.rstr + xo[v]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int d = d0 + u;
+        const bool fresh = self ? (kprev < 0) : (kprev < 0 || d == kprev);
+        if (d < D && !fresh) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const double x = ok[v] ? xv[u][v] : 0.0;
+            if (d == k) xk[v] = x; else ps[v] *= x;
+          }
+        }
+      }
+    }
+    // refreshed cross Grams: X^d(j, i) = sum_m Q_l,j^d(m) F_i^d(m)
+    const int d0 = (self || kprev < 0) ? 0 : kprev;
+    const int d1 = (kprev < 0) ? D : (self ? 0 : kprev + 1);
+    for (int d = d0; d < d1; d++) {
+      const int nd = a.p.nbas[d], nks = (nd + 3) >> 2;
+      const int ja = 16 * g + lr;
+      const double* qa = colp(a.p, d, cl + (ja < Rl ? ja : 0));
+      const double* fb = colp(a.p, d, cF + i0 + (icol ? lr : 0));
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      for (int ks0 = 0; ks0 < nks; ks0 += 8) {
+        double av[8], bv[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+          const int m = 4 * (ks0 + b) + lq;
+          av[b] = (ja < Rl && m < nd) ? qa[m] : 0.0;
+          bv[b] = (icol && m < nd) ? fb[m] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+          if (ks0 + b < nks) acc = mfma_f64_4(av[b], bv[b], acc);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const double x = ok[v] ? acc[v] : 0.0;
+        if (ok[v]) Xl[(long)d * a.rstr + xo[v]] = x;
+        if (d == k) xk[v] = x; else ps[v] *= x;
+      }
+    }
+  }
+  // PS, the own normal-equation matrix or the coefficient partial sum
+  double part = 0.0;
+  double av[4];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const double p = (rows && ok[v]) ? ps[v] : 0.0;
+    if (self) {
+      if (rows && ok[v]) a.bbs[xo[v]] = p;
+    } else {
+      part += p * xk[v] * cq[v] * cfi;
+    }
+    av[v] = p * cq[v];
+  }
+  // P over this wave's rows, tile t: element w is (i = lq + 4w, m = 16t + lr)
+  double* ws = sm + (long)g * NT * 256;
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    v4d c = {0.0, 0.0, 0.0, 0.0};
+    if (rows) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) c = mfma_f64_4(av[v], qk[v][t], c);
+    }
+#pragma unroll
+    for (int w = 0; w < 4; w++) ws[(t * 4 + w) * 64 + lane] = c[w];
+  }
+  if (!self) {
+    const double tot = wg_sum(part, red);
+    if (TID == 0) a.pc[(long)l * GS_PCS + ic] = tot;
+  } else {
     __syncthreads();
   }
-  double acc = 0.0;
-  if (j < Rl)
-    for (int ii = g; ii < ni; ii += 4) {
-      int x = j + (i0 + ii) * Rl;
-      double ps = 1.0;
-      for (int m = 1; m < D; m++) ps *= Xl[(long)((k + m) % D) * a.rstr + x];
-      PS[j + ii * Rl] = ps;
-      if (self)
-        a.bbs[x] = ps;
-      else
-        acc += ps * Xl[(long)k * a.rstr + x] * cq[j] * cf[i0 + ii];
-    }
-  __syncthreads();
-  for (int r = TID >> 5; r < Rl; r += MLCP_NT >> 5)
-    for (int m = TID & 31; m < n; m += 32)
-      Qs[r * n + m] = cq[r] * colp(a.p, k, cl + r)[m];
-  if (!self) {
-    double tot = wg_sum(acc, red);
-    if (TID == 0) a.pc[(long)l * GS_PCS + ic] = tot;
-  }
-  __syncthreads();
   double* P = a.part + (long)l * a.pstr;
-  for (int ii = TID >> 6; ii < ni; ii += MLCP_NT >> 6)
-    for (int m = TID & 63; m < n; m += 64) {
-      double v = 0.0;
-      for (int r = 0; r < Rl; r++) v += PS[r + ii * Rl] * Qs[r * n + m];
-      P[(i0 + ii) * n + m] = v;
-    }
+  const int i = lq + 4 * g;           // thread (lane, g) sums element w = g
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int m = 16 * t + lr;
+    const long e = (long)(t * 4 + g) * 64 + lane;
+    const double v = (sm[e] + sm[NT * 256 + e]) +
+                     (sm[2 * NT * 256 + e] + sm[3 * NT * 256 + e]);
+    if (i < ni && m < n) P[(long)(i0 + i) * n + m] = v;
+  }
 }
 
 // Right-hand side of the Gram-Schmidt step, spread over workgroups (on the
@@ -610,9 +848,9 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
 // order), for a slice of GS_RED_E elements per workgroup.
 __global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k) {
   __shared__ double cl[MLCP_BMAX];
-  if (blockIdx.x % GS_XCDS) return;
-  int job = blockIdx.x / GS_XCDS;
-  int iv = a.iv, R = a.R[iv], n = a.p.nbas[k];
+  if (blockIdx.x % a.xst) return;
+  int job = blockIdx.x / a.xst;
+  int iv = a.iv, R = a.Riv, n = a.p.nbas[k];
   int nch = (R + GS_ICH - 1) / GS_ICH;
   for (int l = TID; l < iv; l += MLCP_NT) {
     double s = 0.0;
@@ -637,40 +875,116 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k) {
 // Solve of the Gram-Schmidt step: regularised solve with the right-hand side
 // of k_gs_reduce, factor replacement, refresh of the own Gram of mode k and
 // the renormalisation c <- c / sqrt(c^T (BB o G_k) c) (ALSUtils.f90:483-531).
+// The system goes from global memory straight into the MFMA tiles of
+// gj_mfma; the solution is transposed into T[m*64 + i] (n*64 doubles of
+// dynamic LDS), from which the row norms, the new factors, the Gram G_k and
+// the renormalisation are formed.
 template <int NX>
 __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
-  extern __shared__ double sm[];
-  __shared__ double red[8];
+  extern __shared__ double T[];
+  __shared__ double red[MLCP_NT + 8];
+  __shared__ double cs[64];
   __shared__ int flag;
-  __shared__ double gjs[512];
-  int iv = a.iv, R = a.R[iv], cF = a.col[iv], n = a.p.nbas[k], xs = n | 1;
-  double* X = sm;              // R*xs
-  double* A = sm + R * xs;     // R*R
-  for (int e = TID; e < R * n; e += MLCP_NT) X[(e / n) * xs + e % n] = a.bjk[e];
-  double pen = *a.pen;
-  for (int e = TID; e < R * R; e += MLCP_NT) {
-    int i = e % R, l = e / R;
-    A[e] = a.bbs[e] + (i == l ? pen : 0.0);
-  }
-  __syncthreads();
-  if (!wg_gj_solve<NX>(A, X, R, n, xs, gjs, &flag)) {
+  __shared__ double gjs[GJ_SC(NX)];
+  const int R = a.Riv, cF = a.cFiv, n = a.p.nbas[k];
+  const int i = TID & 63;
+  const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
+  const double pen = *a.pen;
+  const double* bbs = a.bbs;
+  const double* bjk = a.bjk;
+  v4d w[GjT<NX>::S];
+  gjm_load<NX>(w, R, n,
+               [&](int r, int c) { return bbs[r + c * R] + (r == c ? pen : 0.0); },
+               [&](int r, int c) { return bjk[r * n + c]; });
+  if (!gj_mfma<NX>(w, R, n, gjs, &flag)) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
-  wg_replace_mode(a.p, k, cF, R, X, xs, a.err);
-  double* Gk = a.G + (long)k * a.rstr;
-  wg_gram_rows(X, R, n, xs, Gk);
-  __threadfence_block();
-  __syncthreads();
-  const double* c = a.p.coef + cF;
-  double acc = 0.0;
-  for (int e = TID; e < R * R; e += MLCP_NT) {
-    int j = e % R, i = e / R;
-    acc += a.bbs[e] * Gk[e] * c[i] * c[j];
+  {
+    const int lr = i & 15, lq = i >> 4;
+    const GjGeo geo(R, n);
+#pragma unroll
+    for (int s = 0; s < GjT<NX>::S; s++) {
+      const int t = 4 * s + g;
+      const int rt = t / geo.CT, ct = t % geo.CT;
+      if (t < geo.nt && ct >= geo.RT) {
+        const int x = 16 * (ct - geo.RT) + lr;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          if (x < n) T[x * 64 + 16 * rt + lq + 4 * v] = w[s][v];
+      }
+    }
   }
-  double tot = wg_sum(acc, red);
-  double r = 1.0 / sqrt(fabs(tot));
-  for (int i = TID; i < R; i += MLCP_NT) a.p.coef[cF + i] *= r;
+  __syncthreads();
+  // coef_i = ||X(i,:)||, F_i^k = X(i,:) / coef_i
+  const bool live = i < R;
+  double ss = 0.0;
+  if (live)
+    for (int c = g; c < n; c += 4) ss = fma(T[c * 64 + i], T[c * 64 + i], ss);
+  red[TID] = ss;
+  __syncthreads();
+  const double nrm = sqrt(fabs((red[i] + red[64 + i]) + (red[128 + i] + red[192 + i])));
+  if (g == 0) {
+    cs[i] = live ? nrm : 0.0;
+    if (live && (nrm != nrm || nrm == 0.0)) atomicOr(a.err, 1);
+  }
+  if (live) {
+    const double rn = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    double* f = colp(a.p, k, cF + i);
+    for (int c = g; c < n; c += 4) {
+      const double v = T[c * 64 + i] * rn;
+      T[c * 64 + i] = v;
+      f[c] = v;
+    }
+  }
+  __syncthreads();
+  // G_k = F^k F^kT on the matrix cores: 16 x 16 tiles (tj <= ti) of
+  // T^T T over the basis, tile t of the upper triangle on wave t % 4; each
+  // tile adds its share of c^T (BB o G_k) c (off-diagonal tiles twice)
+  double part = 0.0;
+  double* Gk = a.G + (long)k * a.rstr;
+  const int RT = (R + 15) >> 4, ntile = RT * (RT + 1) / 2;
+  const int lr = i & 15, lq = i >> 4, nks = (n + 3) >> 2;
+  for (int t = g; t < ntile; t += 4) {
+    int tj = 0, rem = t;
+    while (rem >= RT - tj) { rem -= RT - tj; tj++; }
+    const int ti = tj + rem;
+    double bb[4];
+    bool ok[4];
+    const int ic = 16 * ti + lr;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int jr = 16 * tj + lq + 4 * v;
+      ok[v] = jr < R && ic < R;
+      bb[v] = ok[v] ? bbs[ic + jr * R] : 0.0;
+    }
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int ks0 = 0; ks0 < nks; ks0 += 8) {
+      double ta[8], tb[8];
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const int c = 4 * (ks0 + b) + lq;
+        ta[b] = c < n ? T[c * 64 + 16 * tj + lr] : 0.0;
+        tb[b] = c < n ? T[c * 64 + ic] : 0.0;
+      }
+#pragma unroll
+      for (int b = 0; b < 8; b++)
+        if (ks0 + b < nks) acc = mfma_f64_4(ta[b], tb[b], acc);
+    }
+    const double wt = (tj == ti) ? 1.0 : 2.0;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int jr = 16 * tj + lq + 4 * v;
+      if (ok[v]) {
+        Gk[ic + jr * R] = acc[v];
+        if (tj != ti) Gk[jr + ic * R] = acc[v];
+        part += wt * bb[v] * acc[v] * cs[ic] * cs[jr];
+      }
+    }
+  }
+  const double tot = wg_sum(part, red);
+  const double rs = 1.0 / sqrt(fabs(tot));
+  if (live && g == 0) a.p.coef[cF + i] = cs[i] * rs;
 }
 
 // --------------------------------------------------------------------------
@@ -755,7 +1069,7 @@ template <int NX>
 __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
   extern __shared__ double sm[];
   __shared__ int flag;
-  __shared__ double gjs[512];
+  __shared__ double gjs[GJ_SC(NX)];
   int t = blockIdx.x, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof, xs = n | 1;
   double* X = sm;
   double* A = sm + R * xs;

@@ -69,6 +69,13 @@ struct DBuf {
   }
 };
 
+// Register-tile width NX of the templated kernels for a session: the
+// smallest instantiated NX with 4 * NX >= the largest mode size.
+static int nx_of(int nmax) {
+  const int need = (nmax + 3) / 4;
+  return need <= 8 ? 8 : need <= 12 ? 12 : need <= 16 ? 16 : need <= 24 ? 24 : 32;
+}
+
 // Dynamic LDS limit per workgroup: the device's per-block maximum less a
 // margin for the kernels' static LDS.
 static size_t g_lds_limit = 64 * 1024;
@@ -83,9 +90,9 @@ static int setup_kernels() {
 #define NX_KERNELS(kern)                                                    \
   (const void*)kern<8>, (const void*)kern<12>, (const void*)kern<16>,       \
       (const void*)kern<24>, (const void*)kern<32>
-  const void* ks[] = {(const void*)k_op_refresh, (const void*)k_gs_terms,
-                      (const void*)k_lc_rhs,     (const void*)k_op_hadamard,
-                      NX_KERNELS(k_op_solve),    NX_KERNELS(k_gs_solve),
+  const void* ks[] = {(const void*)k_op_refresh,   (const void*)k_lc_rhs,
+                      (const void*)k_op_hadamard,  NX_KERNELS(k_gs_terms),
+                      NX_KERNELS(k_op_solve),      NX_KERNELS(k_gs_solve),
                       NX_KERNELS(k_lc_solve)};
 #undef NX_KERNELS
   size_t lim = (size_t)v;
@@ -371,8 +378,9 @@ static int check_lds(int Rmax, int nmax, int nbloc) {
   (void)nbloc;
   if (lds_ok(2 * n * r * b, "k_op_refresh") ||
       lds_ok((r * n1 + r * r + OPS_JCH * r) * b, "k_op_solve") ||
-      lds_ok((r * n1 + r * r) * b, "k_gs_solve/k_lc_solve") ||
-      lds_ok((r * GS_ICH + r * n1 + GS_ICH * n) * b, "k_gs_terms") ||
+      lds_ok((r * n1 + r * r) * b, "k_lc_solve") ||
+      lds_ok(64 * n * b, "k_gs_solve") ||
+      lds_ok(256 * (size_t)nx_of((int)n) * b, "k_gs_terms") ||
       lds_ok(((size_t)LC_CHUNK * r + (size_t)LC_CHUNK * n) * b, "k_lc_rhs"))
     return -1;
   return 0;
@@ -430,26 +438,30 @@ static int check_err(mlcp_session* s, const char* phase) {
 // --------------------------------------------------------------------------
 // per-kernel probe: HIP events around every launch of one kernel
 
-// Launches the instantiation of a solve kernel whose register tiles hold
-// the right-hand sides of the session's largest mode (n <= 4 * NX).
+// Launches the NX instantiation of a templated kernel for session s.
 #define LAUNCH_NX(kern, grid, lds, ...)                                     \
   do {                                                                    \
-    int nx_ = (s->nmax + 3) / 4;                                          \
-    if (nx_ <= 8)                                                         \
-      hipLaunchKernelGGL((kern<8>), grid, dim3(MLCP_NT), lds, s->st,      \
-                         __VA_ARGS__);                                    \
-    else if (nx_ <= 12)                                                   \
-      hipLaunchKernelGGL((kern<12>), grid, dim3(MLCP_NT), lds, s->st,     \
-                         __VA_ARGS__);                                    \
-    else if (nx_ <= 16)                                                   \
-      hipLaunchKernelGGL((kern<16>), grid, dim3(MLCP_NT), lds, s->st,     \
-                         __VA_ARGS__);                                    \
-    else if (nx_ <= 24)                                                   \
-      hipLaunchKernelGGL((kern<24>), grid, dim3(MLCP_NT), lds, s->st,     \
-                         __VA_ARGS__);                                    \
-    else                                                                  \
-      hipLaunchKernelGGL((kern<32>), grid, dim3(MLCP_NT), lds, s->st,     \
-                         __VA_ARGS__);                                    \
+    switch (nx_of(s->nmax)) {                                             \
+      case 8:                                                             \
+        hipLaunchKernelGGL((kern<8>), grid, dim3(MLCP_NT), lds, s->st,    \
+                           __VA_ARGS__);                                  \
+        break;                                                            \
+      case 12:                                                            \
+        hipLaunchKernelGGL((kern<12>), grid, dim3(MLCP_NT), lds, s->st,   \
+                           __VA_ARGS__);                                  \
+        break;                                                            \
+      case 16:                                                            \
+        hipLaunchKernelGGL((kern<16>), grid, dim3(MLCP_NT), lds, s->st,   \
+                           __VA_ARGS__);                                  \
+        break;                                                            \
+      case 24:                                                            \
+        hipLaunchKernelGGL((kern<24>), grid, dim3(MLCP_NT), lds, s->st,   \
+                           __VA_ARGS__);                                  \
+        break;                                                            \
+      default:                                                            \
+        hipLaunchKernelGGL((kern<32>), grid, dim3(MLCP_NT), lds, s->st,   \
+                           __VA_ARGS__);                                  \
+    }                                                                     \
   } while (0)
 
 enum {
@@ -554,10 +566,25 @@ static int phase_prodhv(mlcp_session* s, int nals) {
   return op_als(s, true, nals, 0, 0.0, false, false);
 }
 
+// Workgroup stride of the Gram-Schmidt kernels: 8 keeps their work on one
+// XCD (workgroups are dealt to the eight XCDs round robin), 1 uses all CUs.
+// MLCP_GS_XCD_STRIDE overrides the default.
+static int gs_stride() {
+  static int v = [] {
+    const char* e = getenv("MLCP_GS_XCD_STRIDE");
+    int x = e ? atoi(e) : 1;
+    return (x == 1 || x == 8) ? x : 1;
+  }();
+  return v;
+}
+
 static GsArgs gs_args(mlcp_session* s, int iv) {
   GsArgs g;
   g.p = s->pool;
   g.iv = iv;
+  g.xst = gs_stride();
+  g.Riv = s->ranks[iv];
+  g.cFiv = s->roff[iv];
   g.col = s->d_colQ.p;
   g.R = s->d_R.p;
   g.pen = s->d_pen.p + iv;
@@ -616,8 +643,8 @@ static int launch_ortho(mlcp_session* s, int nals) {
   hipLaunchKernelGGL(k_penalty, dim3((s->nbloc + 63) / 64), dim3(64), 0,
                      s->st, s->pool, s->d_colQ.p, s->d_R.p, 1.e-10,
                      s->d_pen.p, s->nbloc);
-  size_t lds_t = ((size_t)s->Rmax * GS_ICH + (size_t)s->Rmax * (s->nmax | 1) +
-                  (size_t)GS_ICH * s->nmax) * sizeof(double);
+  // k_gs_terms: four waves' partial P tiles, 4 * (NX / 4) * 256 doubles
+  const size_t lds_t = (size_t)256 * nx_of(s->nmax) * sizeof(double);
   for (int iv = 1; iv < s->nbloc; iv++) {
     GsArgs g = gs_args(s, iv);
     int kprev = -1;
@@ -625,14 +652,13 @@ static int launch_ortho(mlcp_session* s, int nals) {
     for (int it = 0; it < nals; it++)
       for (int k = 0; k < D; k++) {
         pb(s, KID_GS_TERMS);
-        hipLaunchKernelGGL(k_gs_terms, dim3((iv + 1) * nch * GS_XCDS),
-                           dim3(MLCP_NT), lds_t, s->st, g, k, kprev);
+        LAUNCH_NX(k_gs_terms, dim3((iv + 1) * nch * g.xst), lds_t, g, k,
+                  kprev);
         pe(s, KID_GS_TERMS);
         int nred = (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
-        hipLaunchKernelGGL(k_gs_reduce, dim3(nred * GS_XCDS), dim3(MLCP_NT), 0,
+        hipLaunchKernelGGL(k_gs_reduce, dim3(nred * g.xst), dim3(MLCP_NT), 0,
                            s->st, g, k);
-        size_t lds = ((size_t)s->Rmax * (s->nbas[k] | 1) +
-                      (size_t)s->Rmax * s->Rmax) * sizeof(double);
+        size_t lds = (size_t)64 * s->nbas[k] * sizeof(double);
         pb(s, KID_GS_SOLVE);
         LAUNCH_NX(k_gs_solve, dim3(1), lds, g, k);
         pe(s, KID_GS_SOLVE);
