@@ -98,43 +98,48 @@ def kernel_flops(w, kernel):
     raise ValueError(kernel)
 
 
-def cpu_baseline(w, sample_states, ncycle):
+def cpu_baseline(w, sample_states, ncycle, min_seconds=10.0):
     """FP64 oracle (C restatement of the reference CPU path) on one core:
-    each phase of one cycle timed on the first ``sample_states`` states and
-    extrapolated to the whole block by its own scaling (power and reduced
-    H*Q per state; Gram-Schmidt and overlaps per pair; updates per
-    target x source), then times ncycle."""
+    each phase of one cycle timed on the first ``b`` states and extrapolated
+    to the whole block by its own scaling (power and reduced H*Q per state;
+    Gram-Schmidt and overlaps per pair; updates per target x source), then
+    times ncycle.  ``b`` starts at ``sample_states`` and doubles until the
+    sample takes ``min_seconds`` or covers the block."""
     os.environ["OMP_NUM_THREADS"] = "1"
     from mlcp_amd.hoplist import CPBlock
     from mlcp_amd.intertwine import block_shift
     from oracle.orc import C
     Q = w.Q0
-    b = min(sample_states, Q.nbloc)
-    o = Q.offsets
-    sub = CPBlock(Q.nbas, Q.ranks[:b], Q.coef[:o[b]], Q.base[:o[b]],
-                  Q.eigv[:b])
-    es = block_shift(Q.eigv, w.bounds)
-    t = {}
-    t0 = time.perf_counter()
-    for s in range(b):
-        C.als_pow3(w.H, *sub.state(s), w.npow, 1, es)
-    t["power"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    g = C.als_ortho3(sub, w.nals)
-    t["ortho"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for s in range(b):
-        C.prodhv_als3(w.H, *g.state(s), w.nals)
-    t["prodhv"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    C.getoverlaps(g)
-    t["overlaps"] = time.perf_counter() - t0
-    U = np.eye(b)
-    t0 = time.perf_counter()
-    for s in range(b):
-        C.sumlcvec3(*g.state(s), g, U[:, s], w.nals)
-    t["update"] = time.perf_counter() - t0
     B = Q.nbloc
+    o = Q.offsets
+    es = block_shift(Q.eigv, w.bounds)
+    b = min(sample_states, B)
+    while True:
+        sub = CPBlock(Q.nbas, Q.ranks[:b], Q.coef[:o[b]], Q.base[:o[b]],
+                      Q.eigv[:b])
+        t = {}
+        t0 = time.perf_counter()
+        for s in range(b):
+            C.als_pow3(w.H, *sub.state(s), w.npow, 1, es)
+        t["power"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g = C.als_ortho3(sub, w.nals)
+        t["ortho"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for s in range(b):
+            C.prodhv_als3(w.H, *g.state(s), w.nals)
+        t["prodhv"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        C.getoverlaps(g)
+        t["overlaps"] = time.perf_counter() - t0
+        U = np.eye(b)
+        t0 = time.perf_counter()
+        for s in range(b):
+            C.sumlcvec3(*g.state(s), g, U[:, s], w.nals)
+        t["update"] = time.perf_counter() - t0
+        if sum(t.values()) >= min_seconds or b == B:
+            break
+        b = min(B, 2 * b)
     scale = {"power": B / b, "prodhv": B / b,
              "ortho": (B * (B - 1) / 2) / max(1, b * (b - 1) / 2),
              "overlaps": (B * (B + 1)) / (b * (b + 1)) * 2,  # S and QHQ
@@ -159,6 +164,9 @@ def main():
     ap.add_argument("--probe", default="auto")
     ap.add_argument("--cpu-sample-states", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ncycle", type=int, default=0,
+                    help="cycles per step (default: the workload's; fewer "
+                         "only for short profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must "
@@ -172,6 +180,9 @@ def main():
               "qff24": systems.qff24}
     w = makers[args.workload]()
     ref, ref_status = load_reference(args.workload, w)
+    if args.ncycle and args.ncycle != w.ncycle:
+        w.ncycle = args.ncycle
+        ref, ref_status = None, "not compared (--ncycle)"
 
     sess = Session(w.Q0, w.H)
     sess.save()
@@ -193,12 +204,13 @@ def main():
     for _ in range(args.warmup):
         solve()
     phases[:] = 0.0
-    sess.probe(probe)
+    if probe != "none":
+        sess.probe(probe)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         eig = solve()
     dt = time.perf_counter() - t0
-    pms, pn = sess.probe_read()
+    pms, pn = sess.probe_read() if probe != "none" else (0.0, 0)
     sess.probe(None)
     sess.close()
 
@@ -211,7 +223,7 @@ def main():
     value = conv * args.steps / dt
 
     avg_ms = pms / max(pn, 1)
-    fl, by = kernel_counts(w, probe)
+    fl, by = kernel_counts(w, probe) if probe != "none" else (0.0, None)
     achieved = fl / (avg_ms * 1e-3) / 1e12 if pn else None
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"{probe}_{args.workload}_traffic.json")

@@ -15,6 +15,7 @@
 //   download block.
 // Everything runs on one non-default stream of default priority.
 #include "mlcp_kernels.hip"
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cstdlib>
@@ -438,29 +439,25 @@ static int check_err(mlcp_session* s, const char* phase) {
 // --------------------------------------------------------------------------
 // per-kernel probe: HIP events around every launch of one kernel
 
-// Launches the NX instantiation of a templated kernel for session s.
-#define LAUNCH_NX(kern, grid, lds, ...)                                     \
+// Launches the NX instantiation of a templated kernel for session s (timed
+// when kernel kid is being probed).
+#define LAUNCH_NX(kid, kern, grid, lds, ...)                                \
   do {                                                                    \
     switch (nx_of(s->nmax)) {                                             \
       case 8:                                                             \
-        hipLaunchKernelGGL((kern<8>), grid, dim3(MLCP_NT), lds, s->st,    \
-                           __VA_ARGS__);                                  \
+        PLAUNCH(kid, (kern<8>), grid, dim3(MLCP_NT), lds, __VA_ARGS__);   \
         break;                                                            \
       case 12:                                                            \
-        hipLaunchKernelGGL((kern<12>), grid, dim3(MLCP_NT), lds, s->st,   \
-                           __VA_ARGS__);                                  \
+        PLAUNCH(kid, (kern<12>), grid, dim3(MLCP_NT), lds, __VA_ARGS__);  \
         break;                                                            \
       case 16:                                                            \
-        hipLaunchKernelGGL((kern<16>), grid, dim3(MLCP_NT), lds, s->st,   \
-                           __VA_ARGS__);                                  \
+        PLAUNCH(kid, (kern<16>), grid, dim3(MLCP_NT), lds, __VA_ARGS__);  \
         break;                                                            \
       case 24:                                                            \
-        hipLaunchKernelGGL((kern<24>), grid, dim3(MLCP_NT), lds, s->st,   \
-                           __VA_ARGS__);                                  \
+        PLAUNCH(kid, (kern<24>), grid, dim3(MLCP_NT), lds, __VA_ARGS__);  \
         break;                                                            \
       default:                                                            \
-        hipLaunchKernelGGL((kern<32>), grid, dim3(MLCP_NT), lds, s->st,   \
-                           __VA_ARGS__);                                  \
+        PLAUNCH(kid, (kern<32>), grid, dim3(MLCP_NT), lds, __VA_ARGS__);  \
     }                                                                     \
   } while (0)
 
@@ -470,17 +467,30 @@ enum {
   KID_GEP
 };
 
-static void pb(mlcp_session* s, int kid) {
-  if (s->probe_kid == kid && 2 * s->probe_used < (int)s->probe_ev.size())
-    (void)hipEventRecord(s->probe_ev[2 * s->probe_used], s->st);
-}
-
-static void pe(mlcp_session* s, int kid) {
+// Start/stop events of the next launch of kernel kid while it is probed
+// (null otherwise).  The events are recorded by the kernel's own dispatch
+// (hipExtLaunchKernel), so they time the kernel, not the launch gap.
+static void probe_events(mlcp_session* s, int kid, hipEvent_t* e0,
+                         hipEvent_t* e1) {
+  *e0 = *e1 = nullptr;
   if (s->probe_kid == kid && 2 * s->probe_used < (int)s->probe_ev.size()) {
-    (void)hipEventRecord(s->probe_ev[2 * s->probe_used + 1], s->st);
+    *e0 = s->probe_ev[2 * s->probe_used];
+    *e1 = s->probe_ev[2 * s->probe_used + 1];
     s->probe_used++;
   }
 }
+
+// Launch on the session stream, timed if kernel kid is being probed.
+#define PLAUNCH(kid, kern, grid, block, lds, ...)                            \
+  do {                                                                     \
+    hipEvent_t e0_, e1_;                                                   \
+    probe_events(s, kid, &e0_, &e1_);                                      \
+    if (e0_)                                                               \
+      hipExtLaunchKernelGGL(kern, grid, block, lds, s->st, e0_, e1_, 0,    \
+                            __VA_ARGS__);                                  \
+    else                                                                   \
+      hipLaunchKernelGGL(kern, grid, block, lds, s->st, __VA_ARGS__);      \
+  } while (0)
 
 static int probe_collect(mlcp_session* s) {
   if (!s->probe_kid || !s->probe_used) return 0;
@@ -505,10 +515,8 @@ static size_t lds_refresh(mlcp_session* s, int d) {
 static int refresh_all(mlcp_session* s, const OpArgs& a, int do_y) {
   size_t lds = 0;
   for (int d = 0; d < s->ndof; d++) lds = std::max(lds, lds_refresh(s, d));
-  pb(s, KID_OP_REFRESH);
-  hipLaunchKernelGGL(k_op_refresh, dim3(s->njobs_all), dim3(MLCP_NT), lds,
-                     s->st, a, (const int2*)s->jobs_all.p, do_y);
-  pe(s, KID_OP_REFRESH);
+  PLAUNCH(KID_OP_REFRESH, k_op_refresh, dim3(s->njobs_all), dim3(MLCP_NT),
+          lds, a, (const int2*)s->jobs_all.p, do_y);
   HCHK(hipGetLastError());
   return 0;
 }
@@ -524,25 +532,20 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
   if (refresh_all(s, a, 1)) return -1;
   for (int it = 0; it < nitn; it++) {
     for (int k = 0; k < D; k++) {
-      pb(s, KID_OP_HADAMARD);
-      hipLaunchKernelGGL(k_op_hadamard, dim3(s->njobs_had), dim3(s->had_tile),
-                         (size_t)s->nops * s->had_tile * sizeof(double),
-                         s->st, a, (const int2*)s->jobs_had.p, k);
-      pe(s, KID_OP_HADAMARD);
+      PLAUNCH(KID_OP_HADAMARD, k_op_hadamard, dim3(s->njobs_had),
+              dim3(s->had_tile),
+              (size_t)s->nops * s->had_tile * sizeof(double), a,
+              (const int2*)s->jobs_had.p, k);
       size_t lds = ((size_t)s->Rmax * (s->nbas[k] | 1) +
                     (size_t)s->Rmax * s->Rmax + (size_t)OPS_JCH * s->Rmax) *
                    sizeof(double);
-      pb(s, KID_OP_SOLVE);
-      LAUNCH_NX(k_op_solve, dim3(s->nbloc), lds, a, k, ishift, eshift);
-      pe(s, KID_OP_SOLVE);
+      LAUNCH_NX(KID_OP_SOLVE, k_op_solve, dim3(s->nbloc), lds, a, k, ishift,
+                eshift);
       bool last = (it == nitn - 1 && k == D - 1);
       if (!last || renorm) {
-        pb(s, KID_OP_REFRESH);
-        hipLaunchKernelGGL(k_op_refresh, dim3(s->njobs_mode[k]),
-                           dim3(MLCP_NT), lds_refresh(s, k), s->st, a,
-                           (const int2*)s->jobs_mode[k].p,
-                           do_y_each ? 1 : 0);
-        pe(s, KID_OP_REFRESH);
+        PLAUNCH(KID_OP_REFRESH, k_op_refresh, dim3(s->njobs_mode[k]),
+                dim3(MLCP_NT), lds_refresh(s, k), a,
+                (const int2*)s->jobs_mode[k].p, do_y_each ? 1 : 0);
       }
       if (renorm && k == D - 1)
         hipLaunchKernelGGL(k_op_renorm, dim3(s->nbloc), dim3(MLCP_NT), 0,
@@ -651,17 +654,13 @@ static int launch_ortho(mlcp_session* s, int nals) {
     int nch = (s->ranks[iv] + GS_ICH - 1) / GS_ICH;
     for (int it = 0; it < nals; it++)
       for (int k = 0; k < D; k++) {
-        pb(s, KID_GS_TERMS);
-        LAUNCH_NX(k_gs_terms, dim3((iv + 1) * nch * g.xst), lds_t, g, k,
-                  kprev);
-        pe(s, KID_GS_TERMS);
+        LAUNCH_NX(KID_GS_TERMS, k_gs_terms, dim3((iv + 1) * nch * g.xst),
+                  lds_t, g, k, kprev);
         int nred = (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
         hipLaunchKernelGGL(k_gs_reduce, dim3(nred * g.xst), dim3(MLCP_NT), 0,
                            s->st, g, k);
         size_t lds = (size_t)64 * s->nbas[k] * sizeof(double);
-        pb(s, KID_GS_SOLVE);
-        LAUNCH_NX(k_gs_solve, dim3(1), lds, g, k);
-        pe(s, KID_GS_SOLVE);
+        LAUNCH_NX(KID_GS_SOLVE, k_gs_solve, dim3(1), lds, g, k);
         kprev = k;
       }
   }
@@ -673,17 +672,12 @@ static int phase_gep(mlcp_session* s) {
   int B = s->nbloc;
   hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, s->st, s->QHQ.p,
                      2L * B * B, 0.0);
-  pb(s, KID_PAIR_INNER);
-  hipLaunchKernelGGL(k_pair_inner, dim3(s->npj), dim3(MLCP_NT), 0, s->st,
-                     s->pool, (const PairJob*)s->pj.p, s->QHQ.p);
-  pe(s, KID_PAIR_INNER);
+  PLAUNCH(KID_PAIR_INNER, k_pair_inner, dim3(s->npj), dim3(MLCP_NT), 0,
+          s->pool, (const PairJob*)s->pj.p, s->QHQ.p);
   // QHQ at QHQ.p + B*B, S at QHQ.p
-  pb(s, KID_GEP);
-  hipLaunchKernelGGL(k_gep, dim3(1), dim3(1024), 0, s->st,
-                     (const double*)(s->QHQ.p + (size_t)B * B),
-                     (const double*)s->QHQ.p, B, s->ws.p, s->U.p, s->eig.p,
-                     s->err.p);
-  pe(s, KID_GEP);
+  PLAUNCH(KID_GEP, k_gep, dim3(1), dim3(1024), 0,
+          (const double*)(s->QHQ.p + (size_t)B * B), (const double*)s->QHQ.p,
+          B, s->ws.p, s->U.p, s->eig.p, s->err.p);
   HCHK(hipGetLastError());
   return 0;
 }
@@ -720,29 +714,21 @@ static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
   if (s->lcJobs.put(jobs, s->st)) return -1;
   int nj = (int)jobs.size();
   for (int d = 0; d < D; d++) {
-    pb(s, KID_LC_REFRESH);
-    hipLaunchKernelGGL(k_lc_refresh, dim3(nj), dim3(MLCP_NT), 0, s->st, a,
-                       (const int2*)s->lcJobs.p, d);
-    pe(s, KID_LC_REFRESH);
+    PLAUNCH(KID_LC_REFRESH, k_lc_refresh, dim3(nj), dim3(MLCP_NT), 0, a,
+            (const int2*)s->lcJobs.p, d);
   }
   for (int it = 0; it < nitn; it++)
     for (int k = 0; k < D; k++) {
       size_t lds_r = ((size_t)LC_CHUNK * s->Rmax +
                       (size_t)LC_CHUNK * s->nbas[k]) * sizeof(double);
-      pb(s, KID_LC_RHS);
-      hipLaunchKernelGGL(k_lc_rhs, dim3(nj), dim3(MLCP_NT), lds_r, s->st, a,
-                         (const int2*)s->lcJobs.p, k);
-      pe(s, KID_LC_RHS);
+      PLAUNCH(KID_LC_RHS, k_lc_rhs, dim3(nj), dim3(MLCP_NT), lds_r, a,
+              (const int2*)s->lcJobs.p, k);
       size_t lds_s = ((size_t)s->Rmax * (s->nbas[k] | 1) +
                       (size_t)s->Rmax * s->Rmax) * sizeof(double);
-      pb(s, KID_LC_SOLVE);
-      LAUNCH_NX(k_lc_solve, dim3(B), lds_s, a, k);
-      pe(s, KID_LC_SOLVE);
+      LAUNCH_NX(KID_LC_SOLVE, k_lc_solve, dim3(B), lds_s, a, k);
       if (!(it == nitn - 1 && k == D - 1)) {
-        pb(s, KID_LC_REFRESH);
-        hipLaunchKernelGGL(k_lc_refresh, dim3(nj), dim3(MLCP_NT), 0, s->st, a,
-                           (const int2*)s->lcJobs.p, k);
-        pe(s, KID_LC_REFRESH);
+        PLAUNCH(KID_LC_REFRESH, k_lc_refresh, dim3(nj), dim3(MLCP_NT), 0, a,
+                (const int2*)s->lcJobs.p, k);
       }
     }
   if (final_norm)
