@@ -125,6 +125,22 @@ int mlcp_hip_session_rayleigh(mlcp_session *s, double *rq);
 int mlcp_hip_gep(int n, const double *A, const double *B, double *X,
                  double *w);
 
+/* Operator update for the layer above the solved one: the GPU form of
+ * UpdateH's loop over the primitive operators of the solved mode
+ * (Updater.f90:53-107), each operator going through UpdateOperMat
+ * (Updater.f90:109-153).  Operator o is the packed upper triangle at
+ * mat[mptr[o]-1 ..] (1-based, as in the term list) acting on mode opmode[o]
+ * (1-based) of the block's states; the block is in the drop-in layout
+ * (base nD x sum(ranks) column major, coef).  On return
+ *   out[o*m + i + j*(j+1)/2] = <Q_j, X_o Q_i>,  i <= j,  m = nbloc*(nbloc+1)/2,
+ * which is SymPackMat2Vec of QXQ (Utils.f90:559-590), the new packed operator
+ * matrix.  Returns 0, or -1 with mlcp_hip_last_error() set. */
+int mlcp_hip_update_opers(const double *base, const double *coef,
+                          const int *nbas, int ndof, int nbloc,
+                          const int *ranks, int nops, const int *opmode,
+                          const int *mptr, const double *mat, int msize,
+                          double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1345,3 +1345,95 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
     wo[c] = wt[c];
   }
 }
+
+// --------------------------------------------------------------------------
+// Operator update of the layer above the solved one (UpdateH's loop over the
+// primitive operators of a mode, Updater.f90:53-107, each through
+// UpdateOperMat, :109-153): QXQ_o(i,j) = <Q_j, X_o Q_i>, i <= j.
+
+// Y_o(:, c) = X_o q_c^{d_o} for every column c of the block (X_o dense, row
+// major, n x n at xoff[o]; Y_o column major n x ncol at yoff[o]).  Operator
+// o is blockIdx.y.
+__global__ void k_opx_apply(Pool p, int ncol, const int* opmode,
+                            const double* X, const long* xoff, double* Y,
+                            const long* yoff) {
+  const int o = blockIdx.y;
+  const int d = opmode[o], n = p.nbas[d];
+  const double* x = X + xoff[o];
+  double* y = Y + yoff[o];
+  const long tot = (long)ncol * n;
+  for (long e = blockIdx.x * (long)blockDim.x + TID; e < tot;
+       e += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(e / n), m = (int)(e % n);
+    const double* q = colp(p, d, c);
+    double s = 0.0;
+    for (int l = 0; l < n; l++) s = fma(x[(long)m * n + l], q[l], s);
+    y[e] = s;
+  }
+}
+
+#define OPX_G 16   // operators per pass of k_opx_pairs
+
+// One workgroup per state pair (i, j), i <= j:
+//   out[o*npk + i + j(j+1)/2] = sum_{r in i, s in j} c_r c_s
+//       <q_s^{d_o}, X_o q_r^{d_o}> prod_{d != d_o} <q_r^d, q_s^d>
+// for every operator o.  The per-mode Grams of a term pair (r, s) are formed
+// once per pass of OPX_G operators and kept in LDS; each thread sums its
+// term pairs in a fixed order, then the workgroup reduces (wave shuffles,
+// then the four waves in order), so the result does not depend on timing.
+__global__ __launch_bounds__(MLCP_NT) void k_opx_pairs(
+    Pool p, const int* roff, const int* ranks, const int2* pairs, int nops,
+    const int* opmode, const double* Y, const long* yoff, double* out,
+    long npk) {
+  __shared__ double gsh[MLCP_MAXD][MLCP_NT];
+  __shared__ double wsum[OPX_G][MLCP_NT / 64];
+  const int2 pr = pairs[blockIdx.x];
+  const int i = pr.x, j = pr.y;
+  const int Ri = ranks[i], Rj = ranks[j], ci = roff[i], cj = roff[j];
+  const int D = p.ndof;
+  const int lane = TID & 63, w = TID >> 6;
+  for (int o0 = 0; o0 < nops; o0 += OPX_G) {
+    double acc[OPX_G];
+#pragma unroll
+    for (int u = 0; u < OPX_G; u++) acc[u] = 0.0;
+    for (int e = TID; e < Ri * Rj; e += MLCP_NT) {
+      const int r = e / Rj, s = e - r * Rj;
+      for (int d = 0; d < D; d++) {
+        const double* a = colp(p, d, ci + r);
+        const double* b = colp(p, d, cj + s);
+        double g = 0.0;
+        for (int m = 0; m < p.nbas[d]; m++) g = fma(a[m], b[m], g);
+        gsh[d][TID] = g;
+      }
+      const double wgt = p.coef[ci + r] * p.coef[cj + s];
+#pragma unroll
+      for (int u = 0; u < OPX_G; u++) {
+        const int o = o0 + u;
+        if (o < nops) {
+          const int d = opmode[o], n = p.nbas[d];
+          const double* yr = Y + yoff[o] + (long)(ci + r) * n;
+          const double* qs = colp(p, d, cj + s);
+          double t = 0.0;
+          for (int m = 0; m < n; m++) t = fma(qs[m], yr[m], t);
+          double ex = 1.0;
+          for (int dd = 0; dd < D; dd++)
+            if (dd != d) ex *= gsh[dd][TID];
+          acc[u] += wgt * ex * t;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < OPX_G; u++) {
+      double v = acc[u];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if (lane == 0) wsum[u][w] = v;
+    }
+    __syncthreads();
+    if (TID < OPX_G && o0 + TID < nops) {
+      double v = 0.0;
+      for (int q = 0; q < MLCP_NT / 64; q++) v += wsum[TID][q];
+      out[(long)(o0 + TID) * npk + i + (long)j * (j + 1) / 2] = v;
+    }
+    __syncthreads();
+  }
+}

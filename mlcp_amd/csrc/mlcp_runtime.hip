@@ -1021,6 +1021,106 @@ extern "C" int mlcp_hip_session_update(mlcp_session* s, const double* U,
   return check_err(s, "update");
 }
 
+extern "C" int mlcp_hip_update_opers(const double* base, const double* coef,
+                                     const int* nbas, int ndof, int nbloc,
+                                     const int* ranks, int nops,
+                                     const int* opmode, const int* mptr,
+                                     const double* mat, int msize,
+                                     double* out) {
+  g_err.clear();
+  if (ndof < 1 || ndof > MLCP_MAXD) return fail("ndof out of range");
+  if (nbloc < 1 || nbloc > MLCP_BMAX) return fail("nbloc out of range");
+  if (nops < 1) return fail("nops must be >= 1");
+  int nD = 0, Ntot = 0, nmax = 0;
+  for (int d = 0; d < ndof; d++) {
+    if (nbas[d] < 1 || nbas[d] > MLCP_NMAX)
+      return fail("nbas must be in 1.." + std::to_string(MLCP_NMAX));
+    nD += nbas[d];
+    nmax = std::max(nmax, nbas[d]);
+  }
+  std::vector<int> roff(nbloc);
+  for (int s = 0; s < nbloc; s++) {
+    if (ranks[s] < 1) return fail("ranks must be >= 1");
+    roff[s] = Ntot;
+    Ntot += ranks[s];
+  }
+  // dense operators (row major) and the offsets of their Y blocks
+  std::vector<int> om(nops);
+  std::vector<long> xoff(nops), yoff(nops);
+  std::vector<double> X;
+  long ytot = 0;
+  for (int o = 0; o < nops; o++) {
+    if (opmode[o] < 1 || opmode[o] > ndof)
+      return fail("opmode must be in 1..ndof");
+    const int d = opmode[o] - 1, n = nbas[d];
+    const long nq = (long)n * (n + 1) / 2;
+    if (mptr[o] < 1 || mptr[o] - 1 + nq > msize)
+      return fail("operator " + std::to_string(o) + " exceeds mat");
+    const double* ap = mat + (mptr[o] - 1);
+    om[o] = d;
+    xoff[o] = (long)X.size();
+    yoff[o] = ytot;
+    ytot += (long)n * Ntot;
+    for (int a = 0; a < n; a++)
+      for (int b = 0; b < n; b++) {
+        const int lo = std::min(a, b), hi = std::max(a, b);
+        X.push_back(ap[lo + (long)hi * (hi + 1) / 2]);
+      }
+  }
+  std::vector<int2> pairs;
+  for (int j = 0; j < nbloc; j++)
+    for (int i = 0; i <= j; i++) pairs.push_back(make_int2(i, j));
+  const long npk = (long)nbloc * (nbloc + 1) / 2;
+  hipStream_t st = nullptr;
+  HCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } sg{st};
+  DBuf<double> fac, cf, stage, dX, dY, dout;
+  DBuf<int> dR, dro, dom;
+  DBuf<long> dxo, dyo;
+  DBuf<int2> dpairs;
+  if (fac.need((size_t)nD * Ntot) || cf.need(Ntot) ||
+      stage.need((size_t)nD * Ntot + Ntot) || dX.put(X, st) ||
+      dY.need((size_t)ytot) || dout.need((size_t)nops * npk) ||
+      dR.put(std::vector<int>(ranks, ranks + nbloc), st) ||
+      dro.put(roff, st) || dom.put(om, st) || dxo.put(xoff, st) ||
+      dyo.put(yoff, st) || dpairs.put(pairs, st))
+    return -1;
+  Pool pool;
+  pool.ndof = ndof;
+  long off = 0;
+  for (int d = 0; d < ndof; d++) {
+    pool.nbas[d] = nbas[d];
+    pool.moff[d] = off;
+    off += (long)nbas[d] * Ntot;
+  }
+  pool.fac = fac.p;
+  pool.coef = cf.p;
+  const size_t nb = (size_t)nD * Ntot;
+  HCHK(hipMemcpyAsync(stage.p, base, nb * sizeof(double),
+                      hipMemcpyHostToDevice, st));
+  HCHK(hipMemcpyAsync(stage.p + nb, coef, Ntot * sizeof(double),
+                      hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_to_pool, dim3(512), dim3(256), 0, st, stage.p,
+                     stage.p + nb, pool, 0, Ntot, nD);
+  const long napply = (long)Ntot * nmax;
+  hipLaunchKernelGGL(k_opx_apply,
+                     dim3((unsigned)std::min<long>(1024, (napply + 255) / 256),
+                          nops),
+                     dim3(256), 0, st, pool, Ntot, dom.p, dX.p, dxo.p, dY.p,
+                     dyo.p);
+  hipLaunchKernelGGL(k_opx_pairs, dim3((unsigned)pairs.size()), dim3(MLCP_NT),
+                     0, st, pool, dro.p, dR.p, dpairs.p, nops, dom.p, dY.p,
+                     dyo.p, dout.p, npk);
+  HCHK(hipGetLastError());
+  HCHK(hipMemcpyAsync(out, dout.p, (size_t)nops * npk * sizeof(double),
+                      hipMemcpyDeviceToHost, st));
+  HCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
 extern "C" int mlcp_hip_gep(int n, const double* A, const double* Bm,
                             double* X, double* w) {
   g_err.clear();

@@ -97,6 +97,32 @@ def gpu_gep(A: np.ndarray, B: np.ndarray):
     return w, X.T.copy()
 
 
+def update_operators(Q: CPBlock, opmode, ops) -> np.ndarray:
+    """Operator update for the layer above the solved one (UpdateH's loop
+    over the primitive operators of the solved mode, Updater.f90:53-107,
+    each through UpdateOperMat, :109-153), on the GPU.
+
+    ``opmode[o]`` is the 0-based mode of Q that operator o acts on and
+    ``ops[o]`` its packed upper triangle (n x n, column-wise
+    ``i + j(j+1)/2``).  Returns one row per operator: the packed
+    ``nbloc x nbloc`` matrix ``<Q_j, X_o Q_i>`` (i <= j), i.e. the operator
+    in the basis of the block's states."""
+    opmode = np.asarray(opmode, dtype=np.int32) + 1
+    mats = [np.ascontiguousarray(x, dtype=np.float64).ravel() for x in ops]
+    mptr = np.zeros(len(mats), dtype=np.int32)
+    pos = 1
+    for o, x in enumerate(mats):
+        mptr[o] = pos
+        pos += x.size
+    mat = np.concatenate(mats) if mats else np.zeros(0)
+    B = Q.nbloc
+    out = np.zeros((len(mats), B * (B + 1) // 2))
+    check(lib().mlcp_hip_update_opers(
+        Q.base, Q.coef, Q.nbas, len(Q.nbas), B, Q.ranks, len(mats), opmode,
+        mptr, mat, mat.size, out), "mlcp_hip_update_opers")
+    return out
+
+
 class Session:
     """A block and term list resident on the GPU across cycles."""
 

@@ -22,6 +22,8 @@
  *   orc_als_sumlcvec3  compiled reference golden vectors; exact recovery
  *   orc_reduc_als      compiled reference golden vectors; exact recovery
  *   orc_rayleigh       full-tensor Rayleigh quotient
+ *   orc_update_opermat compiled reference UpdateOperMat bitwise; full-tensor
+ *                      <Q_j, (I x .. x X x .. x I) Q_i>
  */
 #include "mlcp_oracle.h"
 
@@ -1006,4 +1008,40 @@ void orc_rayleigh(int ndof, const int *nbas, int T, const int *opid,
   }
   free(wb);
   free(q);
+}
+
+/* UpdateOperMat (Updater.f90:109-153) with PRODXV (:157-186): the operator X
+ * (packed upper triangle, n_imode x n_imode) acting on mode imode (0-based)
+ * is replaced by QXQ(i,j) = PRODVV(Q(j), X Q(i)) for j >= i, packed by
+ * SymPackMat2Vec (Utils.f90:559-590): out[i + j(j+1)/2] = QXQ(i,j). */
+int orc_update_opermat(int ndof, const int *nbas, int nbloc, const int *ranks,
+                       const double *coef, const double *base, int imode,
+                       const double *xmat, double *out) {
+  cpv *q = (cpv *)malloc(sizeof(cpv) * nbloc);
+  int rc = make_block(q, ndof, nbas, nbloc, ranks, (double *)coef,
+                      (double *)base);
+  if (rc || imode < 0 || imode >= ndof) {
+    free(q);
+    return 1;
+  }
+  int nD = q[0].nD, n = nbas[imode];
+  for (int i = 0; i < nbloc; i++) {
+    /* XQ = X * Q(i): CopyWtoV, then dspmv on mode imode of every term */
+    cpv *f = &q[i];
+    double *xc = (double *)malloc(sizeof(double) * f->R);
+    double *xb = (double *)malloc(sizeof(double) * (size_t)f->R * nD);
+    memcpy(xc, f->coef, sizeof(double) * f->R);
+    memcpy(xb, f->base, sizeof(double) * (size_t)f->R * nD);
+    cpv xq;
+    cp_init(&xq, ndof, nbas, f->R, xc, xb);
+    for (int r = 0; r < f->R; r++)
+      orc_dspmv_u(n, 1.0, xmat, B_(f, r, imode), B_(&xq, r, imode));
+#pragma omp parallel for schedule(dynamic)
+    for (int j = i; j < nbloc; j++)
+      out[i + (size_t)j * (j + 1) / 2] = prodvv_ab(&q[j], &xq);
+    free(xb);
+    free(xc);
+  }
+  free(q);
+  return 0;
 }

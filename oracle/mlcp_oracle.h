@@ -112,6 +112,13 @@ void orc_rayleigh(int ndof, const int *nbas, int T, const int *opid,
                   int nbloc, const int *ranks, const double *coef,
                   const double *base, double *rq);
 
+/* UpdateOperMat (Updater.f90:109-153): packed upper QXQ(i,j) =
+ * <Q_j, X Q_i>, j >= i, for X (packed upper) acting on mode imode (0-based).
+ * out holds nbloc*(nbloc+1)/2 doubles. */
+int orc_update_opermat(int ndof, const int *nbas, int nbloc, const int *ranks,
+                       const double *coef, const double *base, int imode,
+                       const double *xmat, double *out);
+
 #ifdef __cplusplus
 }
 #endif

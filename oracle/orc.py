@@ -64,6 +64,8 @@ def lib():
         L.orc_reduc_als.argtypes = [_i, _ip, _i, _dp, _dp, _i, _dp, _dp, _i]
         L.orc_rayleigh.argtypes = _H + [_i, _ip, _dp, _dp, _dp]
         L.orc_rayleigh.restype = None
+        L.orc_update_opermat.argtypes = [_i, _ip, _i, _ip, _dp, _dp, _i, _dp,
+                                         _dp]
         _lib = L
     return _lib
 
@@ -193,6 +195,20 @@ class C:
         return rq
 
 
+    @staticmethod
+    def update_opermat(Q: CPBlock, imode, xpacked):
+        """UpdateOperMat: packed upper <Q_j, X Q_i> (j >= i) for the packed
+        upper operator X on mode imode (0-based)."""
+        B = Q.nbloc
+        out = np.zeros(B * (B + 1) // 2)
+        _chk(lib().orc_update_opermat(len(Q.nbas), Q.nbas, B, Q.ranks, Q.coef,
+                                      Q.base, int(imode),
+                                      np.ascontiguousarray(xpacked,
+                                                           dtype=np.float64),
+                                      out), "update_opermat")
+        return out
+
+
 class Ref:
     """The compiled reference routines, through oracle/_ref/ref_driver."""
 
@@ -278,6 +294,14 @@ class Ref:
     @staticmethod
     def rayleigh(H, Q):
         return Ref.run(8, H, Q)[:Q.nbloc].copy()
+
+    @staticmethod
+    def update_opermat(H, Q):
+        """UpdateOperMat of every operator of the term list on its mode, in
+        operator order: one packed nbloc x nbloc matrix per operator."""
+        v = Ref.run(10, H, Q)
+        m = Q.nbloc * (Q.nbloc + 1) // 2
+        return [v[o * m:(o + 1) * m].copy() for o in range(H.nops)]
 
     @staticmethod
     def dsygv(H, Q, A, B):

@@ -15,7 +15,9 @@
 !   4 GetQHQ3 + GetOverlaps; 5 one Iterate cycle (styp=-1, update as given);
 !   6 UpdateVecs2 with extra = U(nbloc,nbloc); 7 reduc_ALS(G=state 2,
 !   F=state 1); 8 RayleighQuotient2 per state; 9 SolveGenEigval with
-!   extra = [QHQ, S].
+!   extra = [QHQ, S]; 10 UpdateOperMat of every operator of the term list on
+!   its mode, in operator order (each output is the packed nbloc x nbloc
+!   matrix).
 ! Output: int32 status, then real64 arrays depending on the mode.
 !
 ! The term list is turned into a TYPE(Hamiltonian) whose every operator is a
@@ -30,6 +32,7 @@
       use BLOCKUTILS
       use REDUCTION
       use LINALG
+      use UPDATER
       implicit none
       character(len=512) :: fin,fout
       integer :: hdr(12),mode,ndof,nbloc,T,nops,msize,npow,nals,upd
@@ -157,6 +160,14 @@
          call SolveGenEigval(eigv,Smat,QHQ,'V')
          write(11) 0
          write(11) eigv,QHQ
+      CASE (10)
+         write(11) 0
+         DO op=1,nops
+            IF (opmode(op).gt.0) THEN
+               call UpdateOperMat(opmode(op),Q,H%pops(op))
+               write(11) H%pops(op)%mat
+            ENDIF
+         ENDDO
       CASE DEFAULT
          write(11) 1
       END SELECT

@@ -6,8 +6,8 @@
 
 fixtures:  small seeded inputs and the outputs of the compiled reference
            routines (ALS_POW3, PRODHV_ALS3, ALS_ORTHO_3, GetQHQ3/GetOverlaps,
-           UpdateVecs2, reduc_ALS, RayleighQuotient2, one Iterate cycle),
-           saved as ref_<case>.npz.  The C oracle must reproduce them without
+           UpdateVecs2, reduc_ALS, RayleighQuotient2, one Iterate cycle,
+           UpdateOperMat of every operator), saved as ref_<case>.npz.  The C oracle must reproduce them without
            the reference present.
 workloads: the eigenvalues of the FP64 oracle after each cycle of the full
            top-layer solve of the benchmark workloads (ncycle cycles of
@@ -70,6 +70,7 @@ def fixtures():
         QF.coef[:3], QF.base[:3] = F.coef, F.base
         QF.coef[3:], QF.base[3:] = G.coef, G.base
         rc, rb = Ref.reduc_als(H, QF, 4)
+        qxq = np.array(Ref.update_opermat(H, Q))
         np.savez_compressed(
             os.path.join(HERE, f"ref_{name}.npz"),
             nbas=H.nbas, opid=H.opid, mptr=H.mptr, hcoef=H.coef, mat=H.mat,
@@ -79,7 +80,7 @@ def fixtures():
             up_coef=up.coef, up_base=up.base, rq=rq, cyc_coef=cyc.coef,
             cyc_base=cyc.base, cyc_eigv=cyc.eigv, F_coef=F.coef,
             F_base=F.base, G_coef=G.coef, G_base=G.base, red_coef=rc,
-            red_base=rb)
+            red_base=rb, qxq=qxq)
         print("wrote", name)
 
 

@@ -89,3 +89,13 @@ def test_cycle_eigenvalues(fx):
     z, H, Q = fx
     o = C.intertwine(H, Q, 3, 4, True, 1, 5.0)
     assert np.max(np.abs(o.eigv - z["cyc_eigv"])) < 1e-12 * np.abs(z["cyc_eigv"]).max()
+
+
+def test_update_opermat(fx):
+    z, H, Q = fx
+    modes = H.op_mode()
+    for o in range(H.nops):
+        d = int(modes[o])
+        nq = int(H.nbas[d]) * (int(H.nbas[d]) + 1) // 2
+        x = H.mat[H.mptr[o] - 1:H.mptr[o] - 1 + nq]
+        assert np.array_equal(C.update_opermat(Q, d, x), z["qxq"][o]), o

@@ -218,3 +218,31 @@ def test_spectral_range_matches_oracle(hip):
         l, h = C.rayleigh(H, a)[0], C.rayleigh(H, b)[0]
     pad = 0.001 * (h - l)
     assert abs(lo - (l - pad)) < 1e-8 * abs(h) and abs(hi - (h + pad)) < 1e-8 * abs(h)
+
+
+@pytest.mark.parametrize("name,nb,ranks,ncoup", CASES)
+def test_update_operators(hip, name, nb, ranks, ncoup):
+    """UpdateOperMat of every operator of a term list, on the GPU, against
+    the oracle (which matches the compiled reference bitwise)."""
+    from mlcp_amd.intertwine import update_operators
+    from oracle.orc import C
+    say("update operators", name, nb, ranks)
+    H = random_system(95, nb, ncoup=ncoup)
+    Q = random_block(96, nb, ranks)
+    modes = H.op_mode()
+    ops = []
+    for o in range(H.nops):
+        n = int(H.nbas[modes[o]])
+        ops.append(H.mat[H.mptr[o] - 1:H.mptr[o] - 1 + n * (n + 1) // 2])
+    got = update_operators(Q, modes, ops)
+    for o in range(H.nops):
+        want = C.update_opermat(Q, int(modes[o]), ops[o])
+        assert np.max(np.abs(got[o] - want)) < 1e-12 * max(1.0, np.max(np.abs(want))), o
+
+
+def test_update_operators_rejects_bad_mode(hip):
+    from mlcp_amd.intertwine import update_operators
+    say("update operators: bad mode")
+    Q = random_block(97, [4, 3], [2, 2])
+    with pytest.raises(RuntimeError, match="opmode"):
+        update_operators(Q, [5], [np.zeros(10)])

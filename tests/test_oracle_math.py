@@ -174,3 +174,26 @@ def test_harmonic_limit_keeps_exact_eigenstates():
     Q = S.guess_block([g.energies for g in groups], [4] * 8, 5)
     out = C.intertwine(H, Q, 3, 4, True, 1, 0.5 * (2 * Q.eigv[-1] - Q.eigv[-2] + 12))
     assert np.allclose(np.sort(out.eigv), [3, 4, 4, 4, 4, 4, 4, 5], atol=1e-8)
+
+
+@pytest.mark.parametrize("imode", [0, 1, 2])
+def test_update_opermat_full_tensor(imode):
+    """QXQ(i,j) = psi_j . (I x .. x X x .. x I) psi_i for j >= i, packed
+    upper by columns (i + j(j+1)/2); for a symmetric X the packed matrix is
+    the symmetric Q^T X Q."""
+    rng = np.random.default_rng(20 + imode)
+    n = NB[imode]
+    a = rng.normal(size=(n, n))
+    X = a + a.T
+    Q = random_block(21, NB, [2, 3, 2, 4])
+    got = C.update_opermat(Q, imode, pack_upper(X))
+    ops = [np.eye(m) for m in NB]
+    ops[imode] = X
+    big = ops[0]
+    for m in ops[1:]:
+        big = np.kron(big, m)
+    psi = [full_tensor(NB, *Q.state(s)).ravel() for s in range(Q.nbloc)]
+    for i in range(Q.nbloc):
+        for j in range(i, Q.nbloc):
+            want = psi[j] @ big @ psi[i]
+            assert abs(got[i + j * (j + 1) // 2] - want) < 1e-12 * max(1, abs(want))

@@ -149,3 +149,20 @@ def test_cycle_without_update_bitwise():
     o = C.intertwine(H, Q, 2, 3, False, 1, 4.0)
     assert np.array_equal(o.coef, r.coef)
     assert np.array_equal(o.base, r.base)
+
+
+def _op_packed(H, o):
+    d = int(H.op_mode()[o])
+    nq = int(H.nbas[d]) * (int(H.nbas[d]) + 1) // 2
+    return d, H.mat[H.mptr[o] - 1:H.mptr[o] - 1 + nq]
+
+
+@pytest.mark.parametrize("name,nb,ranks,nc", SYSTEMS)
+def test_update_opermat_bitwise(name, nb, ranks, nc):
+    """UpdateOperMat (Updater.f90:109-153) of every operator on its mode."""
+    H = random_system(111, nb, ncoup=nc)
+    Q = random_block(112, nb, ranks)
+    ref = Ref.update_opermat(H, Q)
+    for o in range(H.nops):
+        d, x = _op_packed(H, o)
+        assert np.array_equal(C.update_opermat(Q, d, x), ref[o]), o
