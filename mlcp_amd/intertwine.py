@@ -288,23 +288,59 @@ class SolveResult:
 
 
 def solve(session: Session, eigv0, bounds, ncycle, npow, nals, solvtol,
-          update=True) -> SolveResult:
-    """The cycle loop of SolveHPsi (Solver.f90:389-442) on a resident block:
+          update=True, resfile="none", restart=False) -> SolveResult:
+    """The cycle loop of SolveHPsi (Solver.f90:307-442) on a resident block:
     E-shift from GetBlockShift, one intertwine cycle, relative rms delta
-    convergence test."""
+    convergence test.
+
+    Restart files as in the reference (Solver.f90:340,370,432): with
+    ``restart`` the block, eigenvalues, deltas, spectral bounds and the
+    cycle count are read from ``<resfile>_{1,2}_psi.rst`` (cycles up to the
+    saved one are skipped; the saved ranks must be the session's); without a
+    successful read the initial block is saved as cycle 0.  After every
+    cycle the block is saved, unless ``resfile`` is 'none'."""
+    from .restart import read_psi, save_psi
     eigv = np.asarray(eigv0, dtype=np.float64).copy()
-    res = SolveResult(eigv=eigv)
     nev = eigv.size
-    for i in range(ncycle):
+    bounds = np.asarray(bounds, dtype=np.float64).copy()
+    delta = np.zeros(nev)
+    ist = 0
+    got = None
+    if restart:
+        got = read_psi(resfile, nev, session.Q.nbas)
+    if got is not None and got.success:
+        if not np.array_equal(got.block.ranks, session.Q.ranks):
+            raise ValueError("restart file ranks differ from the session's")
+        session.upload(got.block)
+        ist, bounds, eigv, delta = (got.isavi, got.bounds.copy(),
+                                    got.eigv.copy(), got.delta.copy())
+    else:
+        save_psi(resfile, 0, bounds, eigv, delta, _with_eigv(
+            session.download() if resfile[:4].lower() != "none" else None,
+            eigv))
+    res = SolveResult(eigv=eigv)
+    res.cycles = ist
+    for i in range(1, ncycle + 1):
+        if i <= ist:
+            continue
         eshift = block_shift(eigv, bounds)
         old = eigv.copy()
         eigv = session.cycle(eigv, npow, nals, update, 1, eshift)
         res.history.append(eigv.copy())
-        res.cycles = i + 1
+        res.cycles = i
         delta = (old - eigv) / np.abs(eigv)
         rms = np.sqrt(np.sum(delta ** 2)) / nev
+        if resfile[:4].lower() != "none":
+            save_psi(resfile, i, bounds, eigv, delta,
+                     _with_eigv(session.download(), eigv))
         if rms <= solvtol:
             res.converged = True
             break
     res.eigv = eigv
     return res
+
+
+def _with_eigv(Q, eigv):
+    if Q is not None:
+        Q.eigv = np.asarray(eigv, dtype=np.float64).copy()
+    return Q

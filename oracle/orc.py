@@ -220,13 +220,20 @@ class Ref:
 
     @staticmethod
     def run(mode, H: HopList, Q: CPBlock, npow=0, nals=0, update=0, ishift=1,
-            eshift=0.0, extra=None):
+            eshift=0.0, extra=None, files=None, collect=None):
+        """Runs the driver; ``files`` maps suffixes to bytes written next to
+        its input first, ``collect`` names suffixes of its output to return
+        (then the result is (values, {suffix: bytes}))."""
         extra = np.zeros(0) if extra is None else np.ascontiguousarray(
             extra, dtype=np.float64).ravel(order="F")
         hdr = np.array([Ref.MAGIC, mode, H.ndof, Q.nbloc, H.T, H.nops, H.msize,
                         npow, nals, update, ishift, extra.size], dtype=np.int32)
+        got = {}
         with tempfile.TemporaryDirectory() as td:
             fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+            for suf, data in (files or {}).items():
+                with open(fin + suf, "wb") as f:
+                    f.write(data)
             with open(fin, "wb") as f:
                 f.write(hdr.tobytes())
                 f.write(np.float64(eshift).tobytes())
@@ -239,10 +246,13 @@ class Ref:
             if p.returncode != 0 or not os.path.exists(fout):
                 raise RuntimeError(f"ref_driver failed: {p.stdout}{p.stderr}")
             raw = open(fout, "rb").read()
+            for suf in (collect or []):
+                got[suf] = open(fout + suf, "rb").read()
         status = np.frombuffer(raw[:4], dtype=np.int32)[0]
         if status != 0:
             raise RuntimeError(f"ref_driver status {status}")
-        return np.frombuffer(raw[4:], dtype=np.float64).copy()
+        vals = np.frombuffer(raw[4:], dtype=np.float64).copy()
+        return (vals, got) if collect else vals
 
     @staticmethod
     def _block(vals, Q: CPBlock):
@@ -302,6 +312,30 @@ class Ref:
         v = Ref.run(10, H, Q)
         m = Q.nbloc * (Q.nbloc + 1) // 2
         return [v[o * m:(o + 1) * m].copy() for o in range(H.nops)]
+
+    @staticmethod
+    def save_psi_file(H, Q, isavi, bounds, delta) -> bytes:
+        """SavePsiFile (Restart.f90:759-793) of Q: the file's bytes."""
+        extra = np.concatenate([np.asarray(bounds, float).ravel(),
+                                np.asarray(delta, float).ravel()])
+        _, got = Ref.run(11, H, Q, npow=int(isavi), extra=extra,
+                         collect=[".psi"])
+        return got[".psi"]
+
+    @staticmethod
+    def read_psi_file(H, Q, data: bytes, ranks):
+        """ReadPsiFile (Restart.f90:614-729) of a file's bytes into a block
+        shaped like Q; ``ranks`` are the ranks the file holds."""
+        v = Ref.run(12, H, Q, files={".psi": data})
+        B = Q.nbloc
+        isavi, ok = int(v[0]), bool(v[1])
+        bounds, eigv, delta = v[2:4], v[4:4 + B], v[4 + B:4 + 2 * B]
+        rest = v[4 + 2 * B:]
+        ranks = np.asarray(ranks if ok else Q.ranks, dtype=np.int32)
+        N, nD = int(ranks.sum()), int(np.sum(Q.nbas))
+        out = CPBlock(Q.nbas, ranks, rest[:N].copy(),
+                      rest[N:N + N * nD].reshape(N, nD).copy(), eigv.copy())
+        return isavi, ok, bounds.copy(), eigv.copy(), delta.copy(), out
 
     @staticmethod
     def dsygv(H, Q, A, B):

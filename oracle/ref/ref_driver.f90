@@ -17,7 +17,10 @@
 !   F=state 1); 8 RayleighQuotient2 per state; 9 SolveGenEigval with
 !   extra = [QHQ, S]; 10 UpdateOperMat of every operator of the term list on
 !   its mode, in operator order (each output is the packed nbloc x nbloc
-!   matrix).
+!   matrix); 11 SavePsiFile of the block to <output>.psi with isavi = npow,
+!   extra = [bounds(2), delta(nbloc)]; 12 ReadPsiFile of <input>.psi into
+!   the block, output [isavi, success, bounds(2), eigv, delta] and the block
+!   read.
 ! Output: int32 status, then real64 arrays depending on the mode.
 !
 ! The term list is turned into a TYPE(Hamiltonian) whose every operator is a
@@ -33,6 +36,7 @@
       use REDUCTION
       use LINALG
       use UPDATER
+      use RESTART
       implicit none
       character(len=512) :: fin,fout
       integer :: hdr(12),mode,ndof,nbloc,T,nops,msize,npow,nals,upd
@@ -41,6 +45,10 @@
       integer, allocatable :: nbas(:),ranks(:),opid(:),mptr(:),opmode(:)
       real*8, allocatable  :: hcoef(:),mat(:),coef(:),base(:,:),eigv(:)
       real*8, allocatable  :: extra(:),QHQ(:,:),Smat(:,:),rq(:)
+      real*8  :: bnds(2)
+      real*8, allocatable  :: dlt(:)
+      character(len=64) :: fpsi
+      logical :: ok
       TYPE (Hamiltonian) :: H
       TYPE (CPvec), allocatable :: Q(:),HQ(:)
 
@@ -167,6 +175,28 @@
                call UpdateOperMat(opmode(op),Q,H%pops(op))
                write(11) H%pops(op)%mat
             ENDIF
+         ENDDO
+      CASE (11)
+         bnds=extra(1:2)
+         allocate(dlt(nbloc))
+         dlt=extra(3:2+nbloc)
+         fpsi=trim(fout)//'.psi'
+         call SavePsiFile(npow,bnds,eigv,dlt,Q,fpsi)
+         write(11) 0
+      CASE (12)
+         allocate(dlt(nbloc))
+         bnds=0.d0
+         dlt=0.d0
+         i=-1
+         fpsi=trim(fin)//'.psi'
+         call ReadPsiFile(i,bnds,eigv,dlt,Q,fpsi,ok)
+         write(11) 0
+         write(11) dble(i),merge(1.d0,0.d0,ok),bnds,eigv,dlt
+         DO s=1,nbloc
+            write(11) Q(s)%coef
+         ENDDO
+         DO s=1,nbloc
+            write(11) Q(s)%base
          ENDDO
       CASE DEFAULT
          write(11) 1

@@ -7,7 +7,8 @@
 fixtures:  small seeded inputs and the outputs of the compiled reference
            routines (ALS_POW3, PRODHV_ALS3, ALS_ORTHO_3, GetQHQ3/GetOverlaps,
            UpdateVecs2, reduc_ALS, RayleighQuotient2, one Iterate cycle,
-           UpdateOperMat of every operator), saved as ref_<case>.npz.  The C oracle must reproduce them without
+           UpdateOperMat of every operator), saved as ref_<case>.npz, and
+           the reference's SavePsiFile of the block as ref_<case>_psi.rst.  The C oracle must reproduce them without
            the reference present.
 workloads: the eigenvalues of the FP64 oracle after each cycle of the full
            top-layer solve of the benchmark workloads (ncycle cycles of
@@ -71,6 +72,12 @@ def fixtures():
         QF.coef[3:], QF.base[3:] = G.coef, G.base
         rc, rb = Ref.reduc_als(H, QF, 4)
         qxq = np.array(Ref.update_opermat(H, Q))
+        Qp = Q.copy()
+        Qp.eigv = np.arange(1, Q.nbloc + 1) * 1.25
+        psi = Ref.save_psi_file(H, Qp, 7, [0.5, 99.0],
+                                np.linspace(-1e-3, 1e-3, Q.nbloc))
+        with open(os.path.join(HERE, f"ref_{name}_psi.rst"), "wb") as f:
+            f.write(psi)
         np.savez_compressed(
             os.path.join(HERE, f"ref_{name}.npz"),
             nbas=H.nbas, opid=H.opid, mptr=H.mptr, hcoef=H.coef, mat=H.mat,

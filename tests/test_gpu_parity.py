@@ -246,3 +246,26 @@ def test_update_operators_rejects_bad_mode(hip):
     Q = random_block(97, [4, 3], [2, 2])
     with pytest.raises(RuntimeError, match="opmode"):
         update_operators(Q, [5], [np.zeros(10)])
+
+
+def test_solve_restart_resumes(hip, tmp_path):
+    """SolveHPsi's restart files: two cycles saved, then a restart resumes
+    at cycle 3 and lands where an uninterrupted three-cycle solve lands."""
+    from mlcp_amd.intertwine import Session, solve
+    say("solve restart")
+    nb = [5, 6, 4]
+    H = random_system(98, nb, ncoup=5)
+    Q = random_block(99, nb, [4, 4, 4, 4])
+    Q.eigv[:] = [1.0, 2.0, 3.0, 4.0]
+    bounds = (0.0, 40.0)
+    with Session(Q, H) as s:
+        full = solve(s, Q.eigv, bounds, 3, 2, 3, 0.0)
+    pre = str(tmp_path / "job")
+    with Session(Q, H) as s:
+        first = solve(s, Q.eigv, bounds, 2, 2, 3, 0.0, resfile=pre)
+    assert first.cycles == 2
+    with Session(Q, H) as s:
+        rest = solve(s, Q.eigv, bounds, 3, 2, 3, 0.0, resfile=pre,
+                     restart=True)
+    assert rest.cycles == 3 and len(rest.history) == 1
+    assert np.max(np.abs(rest.eigv - full.eigv)) < 1e-10 * np.abs(full.eigv).max()
