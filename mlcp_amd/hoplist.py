@@ -176,6 +176,59 @@ class HopList:
                        np.array(coefs, dtype=np.float64), np.concatenate(mats))
 
 
+def hoplist_from_hamiltonian(nbas: Sequence[int], opcoef, opindx,
+                             iops: List[np.ndarray], eops: List[np.ndarray],
+                             cops: List[np.ndarray],
+                             pops: List[np.ndarray]) -> HopList:
+    """GetHopList (ModeH.f90:909-1068): the wire format of a mode
+    Hamiltonian held as TYPE(Hamiltonian).
+
+    ``opindx[k][j]`` (term k, mode j) is 0 for the identity, -e for eigen
+    operator e of mode j, i > len(pops) for compound operator i - len(pops),
+    otherwise primitive operator i (1-based into ``pops``); ``iops`` and
+    ``eops`` hold one packed matrix per mode, ``cops``/``pops`` the compound
+    and primitive ones.  Operators are numbered identities (1..D), eigen
+    (D+1..2D), compound, then per mode the distinct primitives of that mode
+    in order of first use (terms outer, modes inner)."""
+    opindx = np.asarray(opindx, dtype=np.int64)
+    nrk, ndof = opindx.shape
+    npop, ncop = len(pops), len(cops)
+    pptr = [[] for _ in range(ndof)]
+    for k in range(nrk):
+        for j in range(ndof):
+            v = int(opindx[k, j])
+            if 1 <= v <= npop and v not in pptr[j]:
+                pptr[j].append(v)
+    opid = np.zeros(nrk * ndof, dtype=np.int32)
+    l = 0
+    for k in range(nrk):
+        psum = 0
+        for j in range(ndof):
+            v = int(opindx[k, j])
+            if v == 0:
+                opid[l] = j + 1
+            elif v < 0:
+                opid[l] = ndof + abs(v)
+            elif v > npop:
+                opid[l] = 2 * ndof + v - npop
+            else:
+                opid[l] = 2 * ndof + ncop + psum + pptr[j].index(v) + 1
+            psum += len(pptr[j])
+            l += 1
+    mats = list(iops) + list(eops) + list(cops)
+    for j in range(ndof):
+        mats += [pops[v - 1] for v in pptr[j]]
+    mptr = np.zeros(len(mats), dtype=np.int32)
+    pos = 1
+    for o, m in enumerate(mats):
+        mptr[o] = pos
+        pos += np.asarray(m).size
+    mat = (np.concatenate([np.asarray(m, dtype=np.float64).ravel()
+                           for m in mats]) if mats else np.zeros(0))
+    return HopList(np.asarray(nbas, dtype=np.int32), opid, mptr,
+                   np.asarray(opcoef, dtype=np.float64), mat)
+
+
 @dataclass
 class CPBlock:
     nbas: np.ndarray     # int32 (ndof,)

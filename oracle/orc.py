@@ -220,10 +220,11 @@ class Ref:
 
     @staticmethod
     def run(mode, H: HopList, Q: CPBlock, npow=0, nals=0, update=0, ishift=1,
-            eshift=0.0, extra=None, files=None, collect=None):
+            eshift=0.0, extra=None, files=None, collect=None, tree=None):
         """Runs the driver; ``files`` maps suffixes to bytes written next to
         its input first, ``collect`` names suffixes of its output to return
-        (then the result is (values, {suffix: bytes}))."""
+        (then the result is (values, {suffix: bytes})), ``tree`` maps paths
+        relative to the driver's working directory to bytes."""
         extra = np.zeros(0) if extra is None else np.ascontiguousarray(
             extra, dtype=np.float64).ravel(order="F")
         hdr = np.array([Ref.MAGIC, mode, H.ndof, Q.nbloc, H.T, H.nops, H.msize,
@@ -234,6 +235,11 @@ class Ref:
             for suf, data in (files or {}).items():
                 with open(fin + suf, "wb") as f:
                     f.write(data)
+            for rel, data in (tree or {}).items():
+                os.makedirs(os.path.dirname(os.path.join(td, rel)),
+                            exist_ok=True)
+                with open(os.path.join(td, rel), "wb") as f:
+                    f.write(data)
             with open(fin, "wb") as f:
                 f.write(hdr.tobytes())
                 f.write(np.float64(eshift).tobytes())
@@ -242,7 +248,7 @@ class Ref:
                 for a in (H.coef, H.mat, Q.coef, Q.base, Q.eigv, extra):
                     f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
             p = subprocess.run([REF_DRIVER, fin, fout], capture_output=True,
-                               text=True, timeout=600)
+                               text=True, timeout=600, cwd=td)
             if p.returncode != 0 or not os.path.exists(fout):
                 raise RuntimeError(f"ref_driver failed: {p.stdout}{p.stderr}")
             raw = open(fout, "rb").read()
@@ -336,6 +342,58 @@ class Ref:
         out = CPBlock(Q.nbas, ranks, rest[:N].copy(),
                       rest[N:N + N * nD].reshape(N, nD).copy(), eigv.copy())
         return isavi, ok, bounds.copy(), eigv.copy(), delta.copy(), out
+
+    @staticmethod
+    def _dummy():
+        nb = np.array([2], dtype=np.int32)
+        H = HopList(nb, np.array([1], np.int32), np.array([1], np.int32),
+                    np.array([1.0]), np.array([1.0, 0.0, 1.0]))
+        Q = CPBlock(nb, np.array([1], np.int32), np.array([1.0]),
+                    np.array([[1.0, 0.0]]), np.array([0.0]))
+        return H, Q
+
+    @staticmethod
+    def read_ff(files: dict, sysid: str, ncp: int, divide: bool):
+        """ReadFFHamiltonian (FFPES.f90:308-406) on the given pes/ files
+        ({relative path: text}): [(qns (n,k), coef (n))] for k = 1..ncp."""
+        H, Q = Ref._dummy()
+        v = Ref.run(13, H, Q, npow=ncp, update=1 if divide else 0,
+                    extra=np.array([float(ord(c)) for c in sysid]),
+                    tree={k: t.encode() for k, t in files.items()})
+        out, pos = [], 0
+        for k in range(1, ncp + 1):
+            n = int(v[pos])
+            qns = v[pos + 1:pos + 1 + n * k].reshape(n, k, order="F")
+            coef = v[pos + 1 + n * k:pos + 1 + n * k + n]
+            out.append((qns.astype(np.int32), coef.copy()))
+            pos += 1 + n * k + n
+        return out
+
+    @staticmethod
+    def get_hoplist(nbas, opcoef, opindx, iops, eops, cops, pops):
+        """GetHopList (ModeH.f90:909-1068) of a Hamiltonian in the
+        reference's own form; operator matrices are (mode 1-based, packed)
+        pairs except iops/eops, which are per mode."""
+        nbas = np.asarray(nbas, dtype=np.int32)
+        D = nbas.size
+        opindx = np.asarray(opindx, dtype=np.int32)
+        modes = list(range(1, D + 1)) * 2 + [m for m, _ in cops] + \
+            [m for m, _ in pops]
+        mats = list(iops) + list(eops) + [x for _, x in cops] + \
+            [x for _, x in pops]
+        mat = np.concatenate([np.asarray(x, float).ravel() for x in mats])
+        H = HopList(nbas, opindx.ravel(), np.array(modes, np.int32),
+                    np.asarray(opcoef, float), mat)
+        Q = CPBlock(nbas, np.array([1], np.int32), np.array([1.0]),
+                    np.zeros((1, int(nbas.sum()))), np.array([0.0]))
+        v = Ref.run(14, H, Q, npow=len(cops), nals=len(pops))
+        n1, n2, n3 = int(v[0]), int(v[1]), int(v[2])
+        pos = 3
+        opid = v[pos:pos + n1].astype(np.int32)
+        mptr = v[pos + n1:pos + n1 + n2].astype(np.int32)
+        m = v[pos + n1 + n2:pos + n1 + n2 + n3].copy()
+        coef = v[pos + n1 + n2 + n3:].copy()
+        return HopList(nbas, opid, mptr, coef, m)
 
     @staticmethod
     def dsygv(H, Q, A, B):

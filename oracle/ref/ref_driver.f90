@@ -20,7 +20,15 @@
 !   matrix); 11 SavePsiFile of the block to <output>.psi with isavi = npow,
 !   extra = [bounds(2), delta(nbloc)]; 12 ReadPsiFile of <input>.psi into
 !   the block, output [isavi, success, bounds(2), eigv, delta] and the block
-!   read.
+!   read; 13 ReadFFHamiltonian of pes/f<k><id>.dat in the working directory
+!   (id = 5 character codes in extra, ncp = npow, divide = update), output
+!   per order k: [n_k, qns(n_k,k), coef(n_k)]; 14 GetHopList of a
+!   Hamiltonian given as opindx (in opid, signed), opcoef (hcoef), and the
+!   operator matrices iops(ndof), eops(ndof), cops(npow), pops(nals) (modes
+!   in mptr, packed triangles concatenated in mat), output
+!   [size(opid), size(mptr), size(mat), opid, mptr, mat, coef].
+! Modes 13 and 14 read no term list: the opid/mptr/mat records carry their
+! own inputs as described.
 ! Output: int32 status, then real64 arrays depending on the mode.
 !
 ! The term list is turned into a TYPE(Hamiltonian) whose every operator is a
@@ -37,6 +45,8 @@
       use LINALG
       use UPDATER
       use RESTART
+      use FFPES
+      use MODEH
       implicit none
       character(len=512) :: fin,fout
       integer :: hdr(12),mode,ndof,nbloc,T,nops,msize,npow,nals,upd
@@ -48,7 +58,11 @@
       real*8  :: bnds(2)
       real*8, allocatable  :: dlt(:)
       character(len=64) :: fpsi
+      character(len=5) :: ffid
       logical :: ok
+      TYPE (Configs), allocatable :: W(:)
+      TYPE (HopList) :: HL
+      integer :: nq2,ncop,npop
       TYPE (Hamiltonian) :: H
       TYPE (CPvec), allocatable :: Q(:),HQ(:)
 
@@ -71,6 +85,61 @@
       read(10) coef,base,eigv
       IF (nextra.gt.0) read(10) extra(1:nextra)
       close(10)
+
+      IF (mode.eq.13 .or. mode.eq.14) THEN
+         open(11,file=trim(fout),access='stream',form='unformatted',&
+              status='replace')
+         IF (mode.eq.13) THEN
+            DO i=1,5
+               ffid(i:i)=achar(nint(extra(i)))
+            ENDDO
+            call ReadFFHamiltonian(W,ffid,npow,upd.ne.0)
+            write(11) 0
+            DO i=1,npow
+               write(11) dble(SIZE(W(i)%coef)),dble(W(i)%qns),W(i)%coef
+            ENDDO
+         ELSE
+            ncop=npow
+            npop=nals
+            allocate(H%opcoef(T),H%opindx(T,ndof),H%iops(ndof),&
+                     H%eops(ndof),H%cops(ncop),H%pops(npop))
+            H%opcoef(:)=hcoef(:)
+            DO i=1,T
+               DO j=1,ndof
+                  H%opindx(i,j)=opid((i-1)*ndof+j)
+               ENDDO
+            ENDDO
+            rst=1
+            DO op=1,nops
+               nq=nbas(mptr(op))*(nbas(mptr(op))+1)/2
+               IF (op.le.ndof) THEN
+                  allocate(H%iops(op)%mat(nq))
+                  H%iops(op)%mat(:)=mat(rst:rst+nq-1)
+                  H%iops(op)%dof=mptr(op)
+               ELSEIF (op.le.2*ndof) THEN
+                  allocate(H%eops(op-ndof)%mat(nq))
+                  H%eops(op-ndof)%mat(:)=mat(rst:rst+nq-1)
+                  H%eops(op-ndof)%dof=mptr(op)
+               ELSEIF (op.le.2*ndof+ncop) THEN
+                  allocate(H%cops(op-2*ndof)%mat(nq))
+                  H%cops(op-2*ndof)%mat(:)=mat(rst:rst+nq-1)
+                  H%cops(op-2*ndof)%dof=mptr(op)
+               ELSE
+                  allocate(H%pops(op-2*ndof-ncop)%mat(nq))
+                  H%pops(op-2*ndof-ncop)%mat(:)=mat(rst:rst+nq-1)
+                  H%pops(op-2*ndof-ncop)%dof=mptr(op)
+               ENDIF
+               rst=rst+nq
+            ENDDO
+            call GetHopList(H,HL,ndof,.TRUE.)
+            write(11) 0
+            write(11) dble(SIZE(HL%opid)),dble(SIZE(HL%mptr)),&
+                      dble(SIZE(HL%mat)),dble(HL%opid),dble(HL%mptr),&
+                      HL%mat,HL%coef
+         ENDIF
+         close(11)
+         STOP
+      ENDIF
 
 !     Term list -> TYPE(Hamiltonian)
       opmode=0

@@ -8,7 +8,9 @@ fixtures:  small seeded inputs and the outputs of the compiled reference
            routines (ALS_POW3, PRODHV_ALS3, ALS_ORTHO_3, GetQHQ3/GetOverlaps,
            UpdateVecs2, reduc_ALS, RayleighQuotient2, one Iterate cycle,
            UpdateOperMat of every operator), saved as ref_<case>.npz, and
-           the reference's SavePsiFile of the block as ref_<case>_psi.rst.  The C oracle must reproduce them without
+           the reference's SavePsiFile of the block as ref_<case>_psi.rst;
+           ReadFFHamiltonian of force-field files and GetHopList of a mode
+           Hamiltonian as ffhl_ref.npz.  The C oracle must reproduce them without
            the reference present.
 workloads: the eigenvalues of the FP64 oracle after each cycle of the full
            top-layer solve of the benchmark workloads (ncycle cycles of
@@ -89,6 +91,80 @@ def fixtures():
             F_base=F.base, G_coef=G.coef, G_base=G.base, red_coef=rc,
             red_base=rb, qxq=qxq)
         print("wrote", name)
+    ff_and_hoplist()
+
+
+def hoplist_case():
+    """A mode Hamiltonian in the reference's own form (identity, eigen,
+    compound and primitive operators; ModeH.f90:909-1068 input)."""
+    from mlcp_amd.hoplist import pack_upper
+    rng = np.random.default_rng(3)
+    nbas = [3, 4, 2]
+    D = len(nbas)
+
+    def rs(n):
+        a = rng.normal(size=(n, n))
+        return pack_upper(a + a.T)
+    iops = [pack_upper(np.eye(n)) for n in nbas]
+    eops = [pack_upper(np.diag(rng.uniform(1, 5, n))) for n in nbas]
+    cops = [(2, rs(4)), (1, rs(3))]
+    pops = [(1, rs(3)), (3, rs(2)), (2, rs(4)), (1, rs(3)), (2, rs(4))]
+    npop = len(pops)
+    prim = {m: [i + 1 for i, (mm, _) in enumerate(pops) if mm == m]
+            for m in range(1, D + 1)}
+    comp = {m: [npop + i + 1 for i, (mm, _) in enumerate(cops) if mm == m]
+            for m in range(1, D + 1)}
+    T = 9
+    opindx = np.zeros((T, D), dtype=np.int32)
+    for k in range(T):
+        for j in range(D):
+            r = rng.integers(0, 4)
+            if r == 1:
+                opindx[k, j] = -(j + 1)
+            elif r == 2 and prim[j + 1]:
+                opindx[k, j] = rng.choice(prim[j + 1])
+            elif r == 3 and comp[j + 1]:
+                opindx[k, j] = comp[j + 1][0]
+    return nbas, rng.normal(size=T), opindx, iops, eops, cops, pops
+
+
+FF_ODD = """  1   1   2  0.5
+  2
+  2 3   -1.25D+01
+3,3,3,4.0
+  1 1 2 7.0
+  1 x 2 9.0
+  4 4 4 1.0
+"""
+
+
+def ff_and_hoplist():
+    """The reference's ReadFFHamiltonian on generated and oddly formatted
+    force-field files, and its GetHopList: ffhl_ref.npz."""
+    import tempfile
+    from mlcp_amd.pesfile import write_pes_files
+    pes = systems.qff_pes(5, 123)
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        files = write_pes_files(pes, "qff05", td)
+        tree = {os.path.relpath(f, td): open(f).read() for f in files}
+    for k in range(1, 5):
+        key = f"pes/f{k}qff05.dat"
+        out[f"ff_text_{k}"] = np.array(tree.get(key, ""))
+    for divide in (0, 1):
+        for k, (q, c) in enumerate(Ref.read_ff(tree, "qff05", 4, divide), 1):
+            out[f"ff_qns_{divide}_{k}"] = q
+            out[f"ff_coef_{divide}_{k}"] = c
+    odd = {"pes/f3odd05.dat": FF_ODD}
+    for k, (q, c) in enumerate(Ref.read_ff(odd, "odd05", 3, True), 1):
+        out[f"odd_qns_{k}"] = q
+        out[f"odd_coef_{k}"] = c
+    nbas, opcoef, opindx, iops, eops, cops, pops = hoplist_case()
+    hl = Ref.get_hoplist(nbas, opcoef, opindx, iops, eops, cops, pops)
+    out.update(hl_opid=hl.opid, hl_mptr=hl.mptr, hl_mat=hl.mat,
+               hl_coef=hl.coef)
+    np.savez_compressed(os.path.join(HERE, "ffhl_ref.npz"), **out)
+    print("wrote ffhl")
 
 
 def solve_oracle(w, ncycle=None):
