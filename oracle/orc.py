@@ -25,6 +25,7 @@ from mlcp_amd.hoplist import CPBlock, HopList
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmlcp_oracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+HOST_DRIVER = os.path.join(HERE, "_ref", "host_driver")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -220,7 +221,8 @@ class Ref:
 
     @staticmethod
     def run(mode, H: HopList, Q: CPBlock, npow=0, nals=0, update=0, ishift=1,
-            eshift=0.0, extra=None, files=None, collect=None, tree=None):
+            eshift=0.0, extra=None, files=None, collect=None, tree=None,
+            driver=None):
         """Runs the driver; ``files`` maps suffixes to bytes written next to
         its input first, ``collect`` names suffixes of its output to return
         (then the result is (values, {suffix: bytes})), ``tree`` maps paths
@@ -247,8 +249,9 @@ class Ref:
                     f.write(np.ascontiguousarray(a, dtype=np.int32).tobytes())
                 for a in (H.coef, H.mat, Q.coef, Q.base, Q.eigv, extra):
                     f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
-            p = subprocess.run([REF_DRIVER, fin, fout], capture_output=True,
-                               text=True, timeout=600, cwd=td)
+            p = subprocess.run([driver or REF_DRIVER, fin, fout],
+                               capture_output=True, text=True, timeout=600,
+                               cwd=td)
             if p.returncode != 0 or not os.path.exists(fout):
                 raise RuntimeError(f"ref_driver failed: {p.stdout}{p.stderr}")
             raw = open(fout, "rb").read()
@@ -342,6 +345,22 @@ class Ref:
         out = CPBlock(Q.nbas, ranks, rest[:N].copy(),
                       rest[N:N + N * nD].reshape(N, nD).copy(), eigv.copy())
         return isavi, ok, bounds.copy(), eigv.copy(), delta.copy(), out
+
+    @staticmethod
+    def host_available() -> bool:
+        return os.path.exists(HOST_DRIVER)
+
+    @staticmethod
+    def host_intertwine(H, Q, npow, nals, update, ishift, eshift):
+        """A Fortran host calling IntertwineCPP as the reference's
+        GPU_Intertwine does (GPUIterate.f90:130-180), linked against the HIP
+        library (oracle/_ref/host_driver): one intertwine cycle of a
+        uniform-rank block on the GPU."""
+        v = Ref.run(0, H, Q, npow=npow, nals=nals, update=int(update),
+                    ishift=ishift, eshift=eshift, driver=HOST_DRIVER)
+        out, rest = Ref._block(v, Q)
+        out.eigv = rest[:Q.nbloc].copy()
+        return out
 
     @staticmethod
     def _dummy():
