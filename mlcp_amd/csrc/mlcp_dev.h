@@ -56,6 +56,20 @@ struct OpArgs {
   const int* kslot_end;
   const int* korder;     // term ids sorted by slot, per mode (offset kord_off)
   const int* kord_off;
+  // grouped term records for the cross-mode Hadamard (per mode k): the
+  // terms of a slot in groups that share the same set of modes carrying the
+  // mode's identity operator; each group multiplies those identity Grams
+  // once, each record its (<= 4) other operators
+  int idop[MLCP_MAXD];   // identity operator of each mode (-1: none)
+  const int* hs_off;     // [ndof+1] into hs_beg/hs_end (one entry per slot)
+  const int* hs_beg;     // [slot] first group
+  const int* hs_end;
+  const unsigned* hg_mask;  // [group] modes whose identity Gram is a factor
+  const int* hg_tb;      // [group] first record
+  const int* hg_te;
+  const double* hr_coef; // [record] term coefficient
+  const int4* hr_ops;    // [record] other operators (-1: none); a term
+                         // with more than four: x = term id, y = -2
   double* M;             // [B][nops][rstr]  M(i,l) = <F_l, h Fold_i>
   double* G;             // [B][ndof][rstr]  G(i,l) = <F_l, F_i>
   double* Y;             // [B][nops][ystr]  Y(:,i) = h Fold_i

@@ -478,7 +478,14 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
 // W_u = sum_{t: op(t,k)=u} coef_t prod_{d!=k} M_{op(t,d)}  and
 // BB = prod_{d!=k} G_d.  job = (s, tile of blockDim.x elements of R x R).
 // Every cached Gram of the element is staged once in LDS (Mc[op][thread])
-// so that the term loop reads LDS instead of re-reading L2.
+// so that the term loop reads LDS instead of re-reading L2.  The terms of a
+// slot come in groups that share the set of modes on which they carry the
+// mode's identity operator: the product of those identity Grams is formed
+// once per group, and each term record multiplies only its remaining
+// (at most four) operators -- for a force field most factors of most terms
+// are identities.  Absent factors point at an LDS entry holding 1, so the
+// record loop is branch free.  Terms with more than four other operators
+// form groups of their own and take the plain product over all modes.
 __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
                                                           const int2* jobs,
                                                           int k) {
@@ -490,26 +497,44 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
   bool live = e < R * R;
   int D = a.p.ndof;
   const double* Ms = a.M + (long)s * a.nops * a.rstr + e;
-  if (live)
+  if (live) {
     for (int op = 0; op < a.nops; op++) Mc[op * E + TID] = Ms[(long)op * a.rstr];
+    Mc[a.nops * E + TID] = 1.0;
+  }
   if (!live) return;
   const double* Gs = a.G + (long)s * D * a.rstr + e;
   double bb = 1.0;
   for (int m = 1; m < D; m++) bb *= Gs[(long)((k + m) % D) * a.rstr];
   a.BB[(long)s * a.rstr + e] = bb;
-  int u0 = a.kslot_off[k], u1 = a.kslot_off[k + 1];
-  const int* ord = a.korder + a.kord_off[k];
   const double* mc = Mc + TID;
-  for (int u = u0; u < u1; u++) {
+  const int h0 = a.hs_off[k], h1 = a.hs_off[k + 1];
+  for (int u = h0; u < h1; u++) {
     double acc = 0.0;
-    for (int q = a.kslot_beg[u]; q < a.kslot_end[u]; q++) {
-      int t = ord[q];
-      const int* to = a.top + (long)t * D;
-      double pr = a.tcoef[t];
-      for (int m = 1; m < D; m++) pr *= mc[to[(k + m) % D] * E];
-      acc += pr;
+    for (int g = a.hs_beg[u]; g < a.hs_end[u]; g++) {
+      const unsigned mask = a.hg_mask[g];
+      const int tb = a.hg_tb[g], te = a.hg_te[g];
+      if (mask >> 31) {
+        // terms with more than four other operators: record x = term id
+        for (int r = tb; r < te; r++) {
+          const int* to = a.top + (long)a.hr_ops[r].x * D;
+          double pr = a.hr_coef[r];
+          for (int m = 1; m < D; m++) pr *= mc[to[(k + m) % D] * E];
+          acc += pr;
+        }
+        continue;
+      }
+      double ip = 1.0;
+      for (int d = 0; d < D; d++)
+        if ((mask >> d) & 1u) ip *= mc[a.idop[d] * E];
+      // absent factors read the entry holding 1, so the loop has no branch
+#pragma unroll 4
+      for (int r = tb; r < te; r++) {
+        const int4 o = a.hr_ops[r];
+        acc += a.hr_coef[r] * ip * ((mc[o.x * E] * mc[o.y * E]) *
+                                    (mc[o.z * E] * mc[o.w * E]));
+      }
     }
-    a.W[((long)s * a.maxslots + (u - u0)) * a.rstr + e] = acc;
+    a.W[((long)s * a.maxslots + (u - h0)) * a.rstr + e] = acc;
   }
 }
 

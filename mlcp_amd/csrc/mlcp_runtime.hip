@@ -136,6 +136,16 @@ struct mlcp_session {
   DBuf<long> d_opoff;
   DBuf<int> d_opmode, d_opcls, d_top, d_kslot_off, d_kslot_op, d_kslot_beg,
       d_kslot_end, d_korder, d_kord_off;
+  // grouped records of the cross-mode Hadamard
+  int idop[MLCP_MAXD];
+  std::vector<int> hs_off, hs_beg, hs_end, hg_tb, hg_te;
+  std::vector<unsigned> hg_mask;
+  std::vector<double> hr_coef;
+  std::vector<int4> hr_ops;
+  DBuf<int> d_hs_off, d_hs_beg, d_hs_end, d_hg_tb, d_hg_te;
+  DBuf<unsigned> d_hg_mask;
+  DBuf<double> d_hr_coef;
+  DBuf<int4> d_hr_ops;
   // per-state tables
   DBuf<int> d_colQ, d_colW, d_R, d_colstate;
   DBuf<double> d_pen;
@@ -259,6 +269,72 @@ static int build_terms(mlcp_session* s, const int* opid, const int* mptr,
   }
   s->kslot_off[D] = (int)s->kslot_op.size();
   s->kord_off[D] = (int)s->korder.size();
+  // grouped records of the cross-mode Hadamard: per mode the canonical
+  // identity operator (the first identity-class operator on it)
+  for (int d = 0; d < MLCP_MAXD; d++) s->idop[d] = -1;
+  for (int op = 0; op < nops; op++)
+    if (s->opcls[op] == 0 && s->opmode[op] >= 0 && s->idop[s->opmode[op]] < 0)
+      s->idop[s->opmode[op]] = op;
+  s->hs_off.assign(D + 1, 0);
+  s->hs_beg.clear();
+  s->hs_end.clear();
+  s->hg_mask.clear();
+  s->hg_tb.clear();
+  s->hg_te.clear();
+  s->hr_coef.clear();
+  s->hr_ops.clear();
+  for (int k = 0; k < D; k++) {
+    s->hs_off[k] = (int)s->hs_beg.size();
+    const int u0 = s->kslot_off[k], u1 = s->kslot_off[k + 1];
+    const int* ord = s->korder.data() + s->kord_off[k];
+    for (int u = u0; u < u1; u++) {
+      s->hs_beg.push_back((int)s->hg_mask.size());
+      // groups of this slot keyed by the identity mask, in first-use order;
+      // terms with more than four other operators form groups of their own
+      // (mask bit 31)
+      std::vector<unsigned> masks;
+      std::vector<std::vector<int>> members;
+      for (int q = s->kslot_beg[u]; q < s->kslot_end[u]; q++) {
+        const int t = ord[q];
+        unsigned m = 0;
+        int nx = 0;
+        for (int d = 0; d < D; d++)
+          if (d != k) {
+            if (s->idop[d] >= 0 && s->top[t * D + d] == s->idop[d])
+              m |= 1u << d;
+            else
+              nx++;
+          }
+        if (nx > 4) m = 1u << 31;
+        auto it = std::find(masks.begin(), masks.end(), m);
+        if (it == masks.end()) {
+          masks.push_back(m);
+          members.push_back({t});
+        } else {
+          members[it - masks.begin()].push_back(t);
+        }
+      }
+      for (size_t g = 0; g < masks.size(); g++) {
+        s->hg_mask.push_back(masks[g]);
+        s->hg_tb.push_back((int)s->hr_coef.size());
+        for (int t : members[g]) {
+          // absent factors point at the LDS entry holding 1 (index nops)
+          int o[4] = {nops, nops, nops, nops}, nx = 0;
+          if (masks[g] >> 31) {
+            o[0] = t;
+          } else {
+            for (int d = 0; d < D; d++)
+              if (d != k && !((masks[g] >> d) & 1u)) o[nx++] = s->top[t * D + d];
+          }
+          s->hr_coef.push_back(hcoef[t]);
+          s->hr_ops.push_back(make_int4(o[0], o[1], o[2], o[3]));
+        }
+        s->hg_te.push_back((int)s->hr_coef.size());
+      }
+      s->hs_end.push_back((int)s->hg_mask.size());
+    }
+  }
+  s->hs_off[D] = (int)s->hs_beg.size();
   std::vector<double> tc(hcoef, hcoef + T);
   hipStream_t st = s->st;
   if (s->opd.put(dense, st) || s->tcoef.put(tc, st) ||
@@ -268,7 +344,11 @@ static int build_terms(mlcp_session* s, const int* opid, const int* mptr,
       s->d_kslot_op.put(s->kslot_op, st) ||
       s->d_kslot_beg.put(s->kslot_beg, st) ||
       s->d_kslot_end.put(s->kslot_end, st) ||
-      s->d_korder.put(s->korder, st) || s->d_kord_off.put(s->kord_off, st))
+      s->d_korder.put(s->korder, st) || s->d_kord_off.put(s->kord_off, st) ||
+      s->d_hs_off.put(s->hs_off, st) || s->d_hs_beg.put(s->hs_beg, st) ||
+      s->d_hs_end.put(s->hs_end, st) || s->d_hg_mask.put(s->hg_mask, st) ||
+      s->d_hg_tb.put(s->hg_tb, st) || s->d_hg_te.put(s->hg_te, st) ||
+      s->d_hr_coef.put(s->hr_coef, st) || s->d_hr_ops.put(s->hr_ops, st))
     return -1;
   return 0;
 }
@@ -329,9 +409,9 @@ static int session_alloc(mlcp_session* s) {
   s->njobs_all = (int)all.size();
   if (s->jobs_all.put(all, st)) return -1;
   // Hadamard tile: elements per workgroup such that every operator's Gram
-  // of the tile fits in LDS
+  // of the tile (and the entry holding 1) fits in LDS
   {
-    size_t per = (size_t)s->nops * sizeof(double);
+    size_t per = (size_t)(s->nops + 1) * sizeof(double);
     int E = MLCP_NT;
     while (E > 64 && per * E > g_lds_limit) E -= 64;
     if (per * E > g_lds_limit)
@@ -411,6 +491,15 @@ static OpArgs op_args(mlcp_session* s, bool work) {
   a.kslot_end = s->d_kslot_end.p;
   a.korder = s->d_korder.p;
   a.kord_off = s->d_kord_off.p;
+  for (int d = 0; d < MLCP_MAXD; d++) a.idop[d] = s->idop[d];
+  a.hs_off = s->d_hs_off.p;
+  a.hs_beg = s->d_hs_beg.p;
+  a.hs_end = s->d_hs_end.p;
+  a.hg_mask = s->d_hg_mask.p;
+  a.hg_tb = s->d_hg_tb.p;
+  a.hg_te = s->d_hg_te.p;
+  a.hr_coef = s->d_hr_coef.p;
+  a.hr_ops = s->d_hr_ops.p;
   a.M = s->M.p;
   a.G = s->G.p;
   a.Y = s->Y.p;
@@ -534,7 +623,7 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
     for (int k = 0; k < D; k++) {
       PLAUNCH(KID_OP_HADAMARD, k_op_hadamard, dim3(s->njobs_had),
               dim3(s->had_tile),
-              (size_t)s->nops * s->had_tile * sizeof(double), a,
+              (size_t)(s->nops + 1) * s->had_tile * sizeof(double), a,
               (const int2*)s->jobs_had.p, k);
       size_t lds = ((size_t)s->Rmax * (s->nbas[k] | 1) +
                     (size_t)s->Rmax * s->Rmax + (size_t)OPS_JCH * s->Rmax) *
