@@ -1026,8 +1026,11 @@ __global__ void k_lc_weights(double* w, const double* U, int ldU,
   }
 }
 
-// X[t][d](i,c) = <F_i^d, O_c^d> for a chunk of old columns; chunk 0 also
-// refreshes G[t][d].  job = (t, chunk).
+// X[t][d](i,c) = <F_i^d, O_c^d> for a chunk of 64 old columns; chunk 0 also
+// refreshes G[t][d].  job = (t, chunk).  The cross Grams are an R x 64 x n
+// matrix product on the FP64 matrix cores: wave g owns rows [16g, 16g+16)
+// of the target and the four 16-column tiles of the chunk, the basis runs
+// in k-steps of 4 (operands loaded eight k-steps ahead).
 __global__ __launch_bounds__(MLCP_NT) void k_lc_refresh(LcArgs a,
                                                          const int2* jobs,
                                                          int d) {
@@ -1035,13 +1038,51 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_refresh(LcArgs a,
   int t = jb.x, ch = jb.y, R = a.R[t], n = a.p.nbas[d];
   int c0 = ch * LC_CHUNK, nc = min(LC_CHUNK, a.ncold - c0);
   double* X = a.X + ((long)t * a.p.ndof + d) * ((long)a.xr * a.ncold);
-  for (int e = TID; e < R * nc; e += MLCP_NT) {
-    int i = e % R, c = c0 + e / R;
-    const double* f = colp(a.p, d, a.col[t] + i);
-    const double* o = colp(a.p, d, a.ocol0 + c);
-    double acc = 0.0;
-    for (int m = 0; m < n; m++) acc += f[m] * o[m];
-    X[(long)c * a.xr + i] = acc;
+  const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
+  const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
+  if (16 * g < R) {
+    const int ia = 16 * g + lr;
+    const double* f = colp(a.p, d, a.col[t] + (ia < R ? ia : 0));
+    const double* o[4];
+    bool cok[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) {
+      const int c = 16 * ct + lr;
+      cok[ct] = c < nc;
+      o[ct] = colp(a.p, d, a.ocol0 + c0 + (cok[ct] ? c : 0));
+    }
+    v4d acc[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) acc[ct] = (v4d){0.0, 0.0, 0.0, 0.0};
+    const int nks = (n + 3) >> 2;
+    for (int ks0 = 0; ks0 < nks; ks0 += 8) {
+      double av[8], bv[8][4];
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const int m = 4 * (ks0 + b) + lq;
+        const bool mok = m < n;
+        av[b] = (mok && ia < R) ? f[m] : 0.0;
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) bv[b][ct] = (mok && cok[ct]) ? o[ct][m] : 0.0;
+      }
+#pragma unroll
+      for (int b = 0; b < 8; b++)
+        if (ks0 + b < nks) {
+#pragma unroll
+          for (int ct = 0; ct < 4; ct++)
+            acc[ct] = mfma_f64_4(av[b], bv[b][ct], acc[ct]);
+        }
+    }
+    // element v of lane l: row 16g + l/16 + 4v, column 16ct + l%16
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) {
+      const int c = c0 + 16 * ct + lr;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = 16 * g + lq + 4 * v;
+        if (i < R && cok[ct]) X[(long)c * a.xr + i] = acc[ct][v];
+      }
+    }
   }
   if (ch == 0) {
     double* G = a.G + ((long)t * a.p.ndof + d) * a.rstr;
