@@ -123,5 +123,8 @@ struct GsArgs {
   double* pc;            // [iv][GS_PCS]  partial Gram-Schmidt coefficients
   double* bbs;           // [rstr] prod_{d!=k} G_d
   double* bjk;           // [pstr] right-hand side of the step
+  double* gjf;           // [GJF_SIZE] block Gauss-Jordan factors of the step
   int* err;
 };
+// factor tiles (pivot block x row block x 256) and the pivot flag
+#define GJF_SIZE (16 * 256 + 64)

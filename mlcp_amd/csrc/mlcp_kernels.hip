@@ -100,16 +100,16 @@ __device__ __forceinline__ void inv4_spd(const double (&P)[4][4],
 }
 
 // Tile slots per wave of gj_mfma: the [A | B] system has at most
-// 4 x (4 + NX/4) tiles of 16 x 16, dealt to the four waves round robin.
+// 4 x (4 + NX/4) tiles of 16 x 16 (R <= 64, n <= 4*NX).
 template <int NX>
 struct GjT {
   static constexpr int S = 4 + NX / 4;   // tiles per wave
-  static constexpr int CP = 64 + 4 * NX; // padded columns of [A | B]
-  static constexpr int BUF = 256 + 4 * CP;
+  static constexpr int CB = 4 * 272;     // pivot column: 4 tiles, row stride 17
+  static constexpr int WS = 192;         // per-wave scratch of the 16 x 16 inverse
 };
-// LDS scratch (doubles) of gj_mfma: two buffers of 4 pivot columns (4 x 64)
-// and 4 pivot rows (4 x CP).
-#define GJ_SC(NX) (2 * GjT<NX>::BUF)
+// LDS scratch (doubles) of gj_mfma: the pivot column tiles, the pivot row
+// tiles and the four waves' scratch.
+#define GJ_SC(NX) (GjT<NX>::CB + 256 * GjT<NX>::S + 4 * GjT<NX>::WS)
 
 // Geometry of the tiled system: row tiles RT = ceil(R/16), column tiles
 // CT = RT + ceil(n/16) (the matrix part first); tile t = rt*CT + ct lives in
@@ -152,28 +152,82 @@ __device__ __forceinline__ void gjm_load(v4d (&w)[GjT<NX>::S], int R, int n,
   }
 }
 
+// Inverse of a 16 x 16 SPD tile held by one wave in the MFMA result layout
+// (element v of lane l is row l/16 + 4v, column l%16): Gauss-Jordan on
+// [P | I] in four steps of four pivots, each pivot block inverted by
+// inv4_spd, the pivot columns and rows exchanged through the wave's own LDS
+// scratch ws (192 doubles; one wave, so no workgroup barrier).  Returns
+// P^-1 in the same layout; bad is set if a pivot is not positive.
+__device__ __forceinline__ v4d inv16_spd(v4d w0, double* ws, bool& bad) {
+  const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
+  v4d w1;
+#pragma unroll
+  for (int v = 0; v < 4; v++) w1[v] = (lq + 4 * v == lr) ? 1.0 : 0.0;
+  double* colB = ws;        // [c*16 + row]
+  double* rowB = ws + 64;   // [r*32 + column of [P | I]]
+#pragma unroll
+  for (int u4 = 0; u4 < 4; u4++) {
+    const bool pcol = lr >= 4 * u4 && lr < 4 * u4 + 4;
+    if (pcol) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) colB[(lr - 4 * u4) * 16 + lq + 4 * v] = w0[v];
+    }
+    rowB[lq * 32 + lr] = w0[u4];
+    rowB[lq * 32 + 16 + lr] = w1[u4];
+    __builtin_amdgcn_wave_barrier();
+    double P[4][4], Q[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c <= r; c++) P[r][c] = colB[c * 16 + 4 * u4 + r];
+    inv4_spd(P, Q, bad);
+    double prow[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      prow[c] = lq == 0 ? Q[0][c] : lq == 1 ? Q[1][c] : lq == 2 ? Q[2][c] : Q[3][c];
+    double m = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) m = fma(prow[c], colB[c * 16 + lr], m);
+    const double mneg = pcol ? 0.0 : -m;
+    const double b0 = rowB[lq * 32 + lr], b1 = rowB[lq * 32 + 16 + lr];
+    double f0 = 0.0, f1 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      f0 = fma(prow[c], rowB[c * 32 + lr], f0);
+      f1 = fma(prow[c], rowB[c * 32 + 16 + lr], f1);
+    }
+    __builtin_amdgcn_wave_barrier();
+    w0[u4] = f0;
+    w1[u4] = f1;
+    w0 = mfma_f64_4(mneg, b0, w0);
+    w1 = mfma_f64_4(mneg, b1, w1);
+  }
+  return w1;
+}
+
 // Solves A X = B (A SPD, R x R, R <= 64; n <= 4*NX right-hand sides) by
-// Gauss-Jordan elimination without pivoting in blocks of four columns on
-// the FP64 matrix cores, with the system [A | B] held as 16 x 16 MFMA tiles
-// in registers (gjm_load).  Per block J = {j0..j0+3}: the owners publish
-// the pivot columns and pivot rows to one of two alternating LDS buffers in
-// sc (GJ_SC(NX) doubles), one workgroup barrier (a buffer is rewritten only
-// after the next barrier, when every wave has read it); every lane then
-// inverts the pivot block P (LDL^T) and forms its multipliers
-// M = A(:,J) P^-1 directly in the MFMA A-operand layout (lane l: row
-// 16rt + l%16, column l/16); the tiles holding rows J first get them
-// replaced by P^-1 W(J,:), then one MFMA per live tile applies
-// W <- W - M W(J,:) (rows J get multiplier zero).
-// Tiles left of the pivot block are dead and skipped.  On exit the
+// block Gauss-Jordan elimination without pivoting on 16 x 16 pivot blocks,
+// the system [A | B] held as 16 x 16 MFMA tiles in registers (gjm_load).
+// Per pivot block J (tile row/column jb): the owners publish the tiles of
+// column jb and of row jb to LDS (sc, GJ_SC(NX) doubles), barrier; every
+// wave inverts the pivot tile P itself (inv16_spd; all waves compute the
+// same bits), forms M(rt)^T = P^-1 A(rt,J)^T for the row tiles it updates
+// (P^-1 is symmetric, so its result-layout elements are its own A operands
+// and M^T in the result layout is the A operand of M), and updates its live
+// tiles (column > jb): rows J become P^-1 W(J,:), the others
+// W <- W - M W(J,:), four MFMA k-steps each; barrier.  On exit the
 // right-hand-side tiles hold X.  Returns false on every thread if a pivot
 // is not positive.
 template <int NX>
 __device__ __forceinline__ bool gj_mfma(v4d (&w)[GjT<NX>::S], int R, int n,
                                         double* sc, int* flag) {
-  constexpr int S = GjT<NX>::S, CP = GjT<NX>::CP, BUF = GjT<NX>::BUF;
+  constexpr int S = GjT<NX>::S, CB = GjT<NX>::CB;
   const int lane = TID & 63, g = __builtin_amdgcn_readfirstlane(TID >> 6);
   const int lr = lane & 15, lq = lane >> 4;
   const GjGeo geo(R, n);
+  double* colT = sc;                          // tile rt: [rt*272 + row*17 + col]
+  double* rowT = sc + CB;                     // tile ct: [ct*256 + v*64 + lane]
+  double* ws = sc + CB + 256 * S + g * GjT<NX>::WS;
   int srt[S], sct[S];
 #pragma unroll
   for (int s = 0; s < S; s++) {
@@ -185,73 +239,185 @@ __device__ __forceinline__ bool gj_mfma(v4d (&w)[GjT<NX>::S], int R, int n,
   const v4d zero = {0.0, 0.0, 0.0, 0.0};
   for (int jb = 0; jb < geo.RT; jb++) {
 #pragma unroll
-    for (int u4 = 0; u4 < 4; u4++) {
-      const int j0 = 16 * jb + 4 * u4;
-      if (j0 < R) {
-        double* colB = sc + (u4 & 1) * BUF;  // [u*64 + row]
-        double* rowB = colB + 256;           // [u*CP + column]
-        const bool pcol = lr >= 4 * u4 && lr < 4 * u4 + 4;
+    for (int s = 0; s < S; s++) {
+      if (sct[s] == jb) {
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-          if (sct[s] == jb && pcol) {
+        for (int v = 0; v < 4; v++)
+          colT[srt[s] * 272 + (lq + 4 * v) * 17 + lr] = w[s][v];
+      }
+      if (srt[s] == jb && sct[s] > jb) {
 #pragma unroll
-            for (int v = 0; v < 4; v++)
-              colB[(lr - 4 * u4) * 64 + 16 * srt[s] + lq + 4 * v] = w[s][v];
-          }
-          if (srt[s] == jb) rowB[lq * CP + 16 * sct[s] + lr] = w[s][u4];
-        }
-        __syncthreads();
-        double P[4][4], Q[4][4];
+        for (int v = 0; v < 4; v++) rowT[sct[s] * 256 + v * 64 + lane] = w[s][v];
+      }
+    }
+    __syncthreads();
+    v4d pv;
 #pragma unroll
-        for (int r = 0; r < 4; r++)
+    for (int v = 0; v < 4; v++) pv[v] = colT[jb * 272 + (lq + 4 * v) * 17 + lr];
+    const v4d pinv = inv16_spd(pv, ws, bad);
+    // M(rt)^T for the row tiles of this wave's live tiles
+    v4d mt[4];
 #pragma unroll
-          for (int c = 0; c <= r; c++) P[r][c] = colB[c * 64 + j0 + r];
-        inv4_spd(P, Q, bad);
-        // row l/16 of P^-1 (P^-1 is symmetric)
-        double prow[4];
+    for (int rt = 0; rt < 4; rt++) {
+      bool need = false;
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-          prow[c] = lq == 0 ? Q[0][c] : lq == 1 ? Q[1][c] : lq == 2 ? Q[2][c] : Q[3][c];
-        // multipliers -M(16rt + l%16, l/16) in the A-operand layout,
-        // M = A(:,J) P^-1, zero on the pivot rows
-        double mneg[4];
+      for (int s = 0; s < S; s++) need |= (srt[s] == rt && sct[s] > jb);
+      mt[rt] = zero;
+      if (need && rt != jb) {
+        const double* ct_ = colT + rt * 272 + lr * 17 + lq;
 #pragma unroll
-        for (int rt = 0; rt < 4; rt++) {
-          double m = 0.0;
-          if (rt < geo.RT) {
+        for (int ks = 0; ks < 4; ks++) mt[rt] = mfma_f64_4(pinv[ks], ct_[4 * ks], mt[rt]);
+      }
+    }
 #pragma unroll
-            for (int c = 0; c < 4; c++)
-              m = fma(prow[c], colB[c * 64 + 16 * rt + lr], m);
-          }
-          mneg[rt] = (rt == jb && pcol) ? 0.0 : -m;
-        }
+    for (int s = 0; s < S; s++) {
+      const int rt = srt[s], ct = sct[s];
+      if (ct > jb) {
+        const double* rb = rowT + ct * 256 + lane;
+        double bk[4];
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-          const int rt = srt[s], ct = sct[s];
-          if (ct >= jb) {
-            const double* rb = rowB + 16 * ct + lr;
-            const double bro = rb[lq * CP];
-            if (rt == jb) {
-              // pivot rows: W(J, c) <- P^-1 W(J, c) (the update below
-              // leaves them, their multipliers being zero)
-              double f = 0.0;
+        for (int ks = 0; ks < 4; ks++) bk[ks] = rb[ks * 64];
+        if (rt == jb) {
+          v4d acc = zero;
 #pragma unroll
-              for (int c = 0; c < 4; c++) f = fma(prow[c], rb[c * CP], f);
-              w[s][u4] = f;
-            }
-            const double m = rt == 0 ? mneg[0] : rt == 1 ? mneg[1]
-                           : rt == 2 ? mneg[2] : mneg[3];
-            w[s] = mfma_f64_4(m, bro, w[s]);
-          }
+          for (int ks = 0; ks < 4; ks++) acc = mfma_f64_4(pinv[ks], bk[ks], acc);
+          w[s] = acc;
+        } else {
+          const v4d m = rt == 0 ? mt[0] : rt == 1 ? mt[1] : rt == 2 ? mt[2] : mt[3];
+#pragma unroll
+          for (int ks = 0; ks < 4; ks++) w[s] = mfma_f64_4(-m[ks], bk[ks], w[s]);
         }
       }
     }
+    __syncthreads();
   }
   if (TID == 0) *flag = bad ? 1 : 0;
   __syncthreads();
   bool ok = (*flag == 0);
   __syncthreads();
   return ok;
+}
+
+__device__ __forceinline__ int clampi(int x, int hi) {
+  return x < 0 ? 0 : (x > hi ? hi : x);
+}
+
+// *p if ok, else 0, loaded by every lane (p must be a valid address of a
+// finite value): the value feeds an FMA with a 0/1 selector, so the compiler
+// cannot sink the load into a branch and a group of such loads is in flight
+// together
+__device__ __forceinline__ double ld_or0(const double* p, bool ok) {
+  return fma(ok ? 1.0 : 0.0, *p, 0.0);
+}
+
+// a[k & 3] through bit masks (no branches, whatever the compiler prefers)
+__device__ __forceinline__ double sel4(double a0, double a1, double a2,
+                                       double a3, int k) {
+  const unsigned long long m0 = (k & 1) ? ~0ull : 0ull, m1 = (k & 2) ? ~0ull : 0ull;
+  const unsigned long long b0 = __double_as_longlong(a0), b1 = __double_as_longlong(a1);
+  const unsigned long long b2 = __double_as_longlong(a2), b3 = __double_as_longlong(a3);
+  const unsigned long long lo = (b0 & ~m0) | (b1 & m0), hi = (b2 & ~m0) | (b3 & m0);
+  return __longlong_as_double((lo & ~m1) | (hi & m1));
+}
+
+// Gauss-Jordan factors of the SPD R x R matrix A(r, c) + pen delta_rc
+// (R <= 16 RTM <= 64), computed by wave 0 alone (no workgroup barrier; the
+// other waves return).  Step u (pivot rows/columns J = 4u..4u+3, pivot block P) is
+//   W <- W + E_u W(J, :),  E_u(i, :) = -W(i, J) P^-1 (i not in J),
+//                          E_u(J_r, :) = P^-1(r, :) - e_r,
+// i.e. rows J become P^-1 W(J, :) and the others lose W(i, J) P^-1 W(J, :).
+// P^-1 comes from inv4_spd.  The matrix is held as 16 x 16 tiles in the
+// MFMA result layout (element v of lane l: row 16rt + l/16 + 4v, column
+// 16ct + l%16); E_u for row tile rt is stored in the A-operand layout
+// (lane l: row 16rt + l%16, column l/16) at f[(u*4 + rt)*64 + lane], and
+// W(J, ct-tile) is the B operand of the update held by the same lane as
+// element u%4 of tile (u/4, ct), so applying the steps to right-hand sides
+// needs no lane exchange.  f[16*256] = 1 if a pivot is not positive, else
+// 0.  sc: GJF_SC doubles of LDS.
+#define GJF_SC 256
+template <int RTM, typename FA>
+__device__ __forceinline__ void gj_factor(FA A, double pen, int R, double* f,
+                                          double* sc) {
+  if (TID >= 64) return;
+  const int lane = TID, lr = lane & 15, lq = lane >> 4;
+  const int RT = (R + 15) >> 4, nst = (R + 3) >> 2;
+  const v4d zero = {0.0, 0.0, 0.0, 0.0};
+  // tiles w[rt][ct] (RT <= RTM); every lane loads a clamped (finite) entry,
+  // selected through an FMA, so all loads are in flight together
+  v4d w[RTM][RTM];
+#pragma unroll
+  for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+    for (int ct = 0; ct < RTM; ct++) {
+      w[rt][ct] = zero;
+      if (rt < RT && ct < RT) {   // uniform
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int i = 16 * rt + lq + 4 * v, c = 16 * ct + lr;
+          const double a = A(clampi(i, R - 1), clampi(c, R - 1));
+          const bool in = i < R && c < R;
+          const double idv = (!in && i == c) ? 1.0 : 0.0;
+          w[rt][ct][v] = fma(in ? 1.0 : 0.0, a + (i == c ? pen : 0.0), idv);
+        }
+      }
+    }
+  double* colB = sc;   // [c*64 + row]: the four pivot columns
+  bool bad = false;
+#pragma unroll
+  for (int u = 0; u < 4 * RTM; u++) {
+    if (u < nst) {   // uniform
+      const int jb = u >> 2, u4 = u & 3;
+      const bool pcol = (lr >> 2) == u4;
+#pragma unroll
+      for (int rt = 0; rt < RTM; rt++)
+        if (rt < RT && pcol) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) colB[(lr & 3) * 64 + 16 * rt + lq + 4 * v] = w[rt][jb][v];
+        }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double P[4][4], Q[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c <= r; c++) P[r][c] = colB[c * 64 + 4 * u + r];
+      double cb[RTM][4];
+#pragma unroll
+      for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) cb[rt][c] = colB[c * 64 + 16 * rt + lr];
+      inv4_spd(P, Q, bad);
+      // row l/16 of P^-1 (symmetric)
+      double prow[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) prow[c] = sel4(Q[0][c], Q[1][c], Q[2][c], Q[3][c], lq);
+      const double pj = sel4(prow[0], prow[1], prow[2], prow[3], lr & 3) -
+                        ((lr & 3) == lq ? 1.0 : 0.0);
+      double e[RTM];
+#pragma unroll
+      for (int rt = 0; rt < RTM; rt++) {
+        double m = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) m = fma(prow[c], cb[rt][c], m);
+        e[rt] = (rt == jb && pcol) ? pj : -m;
+        if (rt < RT) f[(u * 4 + rt) * 64 + lane] = e[rt];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int ct = 0; ct < RTM; ct++) {
+        if (ct >= jb && ct < RT) {
+          const double bro = w[jb][ct][u4];
+#pragma unroll
+          for (int rt = 0; rt < RTM; rt++)
+            if (rt < RT) w[rt][ct] = mfma_f64_4(e[rt], bro, w[rt][ct]);
+        }
+      }
+    }
+  }
+  if (lane == 0) f[16 * 256] = bad ? 1.0 : 0.0;
 }
 
 // Solves A X = B for an SPD R x R matrix A (column major, LDS) and n
@@ -870,12 +1036,24 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
 // Right-hand side of the Gram-Schmidt step, spread over workgroups (on the
 // same XCD as k_gs_terms): coef_l = -sum of the partial sums and
 // bjk = sum_{l<iv} coef_l P_l + P_iv (previous states first, ALSUtils.f90
-// order), for a slice of GS_RED_E elements per workgroup.
+// order), for a slice of MLCP_NT elements per workgroup.  Workgroup 0
+// instead factors the step's system matrix BB + pen I (gj_factor, rank
+// R <= 16 RTM), so k_gs_solve only applies the elimination.
+template <int RTM>
 __global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k) {
   __shared__ double cl[MLCP_BMAX];
+  __shared__ double fsc[GJF_SC];
   if (blockIdx.x % a.xst) return;
   int job = blockIdx.x / a.xst;
   int iv = a.iv, R = a.Riv, n = a.p.nbas[k];
+  if (job == 0) {
+    // factors of the step's system matrix BB + pen I for k_gs_solve
+    const double* bb = a.bbs;
+    gj_factor<RTM>([&](int r, int c) { return bb[r + c * R]; }, *a.pen, R,
+                   a.gjf, fsc);
+    return;
+  }
+  job -= 1;
   int nch = (R + GS_ICH - 1) / GS_ICH;
   for (int l = TID; l < iv; l += MLCP_NT) {
     double s = 0.0;
@@ -885,131 +1063,213 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k) {
   __syncthreads();
   int e = job * MLCP_NT + TID;
   if (e >= R * n) return;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  // sixteen independent chains: sixteen partials in flight per round trip
+  constexpr int U = 16;
+  double sv[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) sv[u] = 0.0;
+  const double* pe = a.part + e;
   int l = 0;
-  for (; l + 3 < iv; l += 4) {
-    s0 += cl[l] * a.part[(long)l * a.pstr + e];
-    s1 += cl[l + 1] * a.part[(long)(l + 1) * a.pstr + e];
-    s2 += cl[l + 2] * a.part[(long)(l + 2) * a.pstr + e];
-    s3 += cl[l + 3] * a.part[(long)(l + 3) * a.pstr + e];
+  for (; l + U <= iv; l += U) {
+    double pv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) pv[u] = pe[(long)(l + u) * a.pstr];
+#pragma unroll
+    for (int u = 0; u < U; u++) sv[u] = fma(cl[l + u], pv[u], sv[u]);
   }
-  for (; l < iv; l++) s0 += cl[l] * a.part[(long)l * a.pstr + e];
-  a.bjk[e] = ((s0 + s1) + (s2 + s3)) + a.part[(long)iv * a.pstr + e];
+  {
+    // the last iv - l < U partials, from clamped rows with weight zero
+    double pv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) pv[u] = pe[(long)min(l + u, iv) * a.pstr];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (l + u < iv) sv[u] = fma(cl[l + u], pv[u], sv[u]);
+  }
+#pragma unroll
+  for (int w = U / 2; w > 0; w >>= 1)
+#pragma unroll
+    for (int u = 0; u < w; u++) sv[u] += sv[u + w];
+  a.bjk[e] = sv[0] + a.part[(long)iv * a.pstr + e];
 }
 
 // Solve of the Gram-Schmidt step: regularised solve with the right-hand side
 // of k_gs_reduce, factor replacement, refresh of the own Gram of mode k and
 // the renormalisation c <- c / sqrt(c^T (BB o G_k) c) (ALSUtils.f90:483-531).
-// The system goes from global memory straight into the MFMA tiles of
-// gj_mfma; the solution is transposed into T[m*64 + i] (n*64 doubles of
-// dynamic LDS), from which the row norms, the new factors, the Gram G_k and
-// the renormalisation are formed.
-template <int NX>
+// The system matrix was factored by k_gs_reduce (gj_factor), so the solve
+// is the elimination applied to the right-hand side: wave g holds the 16-column tiles cb = g, g+4, ... of B for all row
+// tiles in registers and applies the ceil(R/4) steps with one MFMA per
+// tile each, needing no other wave.  The solution goes to T[m*65 + i] (65 n doubles of dynamic
+// LDS); the Gram X X^T over the basis (MFMA tiles, tj <= ti on wave t % 4)
+// gives the row norms coef_i = sqrt(G(i,i)), the new factors
+// F_i^k = X(i,:) / coef_i, G_k = G(i,j) / (coef_i coef_j) and the
+// renormalisation.
+template <int NX, int RTM>
 __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
+  constexpr int NCW = (NX / 4 + 3) / 4;   // column tiles of B per wave
   extern __shared__ double T[];
-  __shared__ double red[MLCP_NT + 8];
+  __shared__ double red[8];
   __shared__ double cs[64];
-  __shared__ int flag;
-  __shared__ double gjs[GJ_SC(NX)];
   const int R = a.Riv, cF = a.cFiv, n = a.p.nbas[k];
-  const int i = TID & 63;
+  const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
   const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
-  const double pen = *a.pen;
-  const double* bbs = a.bbs;
+  const int RT = (R + 15) >> 4, CB = (n + 15) >> 4;
+  const double* f = a.gjf;
   const double* bjk = a.bjk;
-  v4d w[GjT<NX>::S];
-  gjm_load<NX>(w, R, n,
-               [&](int r, int c) { return bbs[r + c * R] + (r == c ? pen : 0.0); },
-               [&](int r, int c) { return bjk[r * n + c]; });
-  if (!gj_mfma<NX>(w, R, n, gjs, &flag)) {
+  const v4d zero = {0.0, 0.0, 0.0, 0.0};
+  // right-hand side tiles, element v of lane l: row 16rt + l/16 + 4v,
+  // column 16cb + l%16
+  // (all loads from clamped addresses, issued together: one round trip)
+  v4d x[NCW][RTM];
+#pragma unroll
+  for (int j = 0; j < NCW; j++) {
+    const int m = 16 * (g + 4 * j) + lr;
+#pragma unroll
+    for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = 16 * rt + lq + 4 * v;
+        x[j][rt][v] = ld_or0(bjk + clampi(i, R - 1) * n + clampi(m, n - 1),
+                             i < R && m < n);
+      }
+  }
+  // elimination steps E_u (u < ceil(R/4)) for each row tile, A-operand
+  // layout (uniform guards: no waits between the loads)
+  const int nst = (R + 3) >> 2;
+  double e[4 * RTM][RTM];
+#pragma unroll
+  for (int u = 0; u < 4 * RTM; u++)
+#pragma unroll
+    for (int rt = 0; rt < RTM; rt++) {
+      e[u][rt] = 0.0;
+      if (u < nst && rt < RT) e[u][rt] = f[(u * 4 + rt) * 64 + lane];
+    }
+  if (f[16 * 256] != 0.0) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
-  {
-    const int lr = i & 15, lq = i >> 4;
-    const GjGeo geo(R, n);
+  // X = B after every step W <- W + E_u W(J_u, :); W(J_u, :) of column tile
+  // j is element u%4 of this lane's tile (u/4, j)
 #pragma unroll
-    for (int s = 0; s < GjT<NX>::S; s++) {
-      const int t = 4 * s + g;
-      const int rt = t / geo.CT, ct = t % geo.CT;
-      if (t < geo.nt && ct >= geo.RT) {
-        const int x = 16 * (ct - geo.RT) + lr;
+  for (int u = 0; u < 4 * RTM; u++) {
+    const int jb = u >> 2, u4 = u & 3;
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-          if (x < n) T[x * 64 + 16 * rt + lq + 4 * v] = w[s][v];
+    for (int j = 0; j < NCW; j++) {
+      if (u < nst && g + 4 * j < CB) {   // uniform
+        const double bro = x[j][jb][u4];
+#pragma unroll
+        for (int rt = 0; rt < RTM; rt++)
+          if (rt < RT) x[j][rt] = mfma_f64_4(e[u][rt], bro, x[j][rt]);
+      }
+    }
+  }
+  // X -> T[m*65 + i]
+#pragma unroll
+  for (int j = 0; j < NCW; j++) {
+    const int m = 16 * (g + 4 * j) + lr;
+    if (m < n) {
+#pragma unroll
+      for (int rt = 0; rt < RTM; rt++)
+        if (rt < RT) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) T[m * 65 + 16 * rt + lq + 4 * v] = x[j][rt][v];
+        }
+    }
+  }
+  __syncthreads();
+  // Gram G = X X^T on the matrix cores, tile t of the upper triangle on
+  // wave t % 4 (at most 10 tiles: three per wave)
+  const int ntile = RT * (RT + 1) / 2, nks = (n + 3) >> 2;
+  v4d gt[3];
+  int gtj[3], gti[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const int t = g + 4 * q;
+    gt[q] = zero;
+    gtj[q] = -1;
+    gti[q] = -1;
+    if (t < ntile) {
+      int tj = 0, rem = t;
+      while (rem >= RT - tj) { rem -= RT - tj; tj++; }
+      const int ti = tj + rem;
+      gtj[q] = tj;
+      gti[q] = ti;
+      v4d acc = zero;
+      for (int ks0 = 0; ks0 < nks; ks0 += 8) {
+        double ta[8], tb[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+          const int c = 4 * (ks0 + b) + lq;
+          ta[b] = c < n ? T[c * 65 + 16 * tj + lr] : 0.0;
+          tb[b] = c < n ? T[c * 65 + 16 * ti + lr] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+          if (ks0 + b < nks) acc = mfma_f64_4(ta[b], tb[b], acc);
+      }
+      gt[q] = acc;
+      // element v: row j = 16tj + l/16 + 4v, column i = 16ti + l%16
+      if (tj == ti) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int i = 16 * ti + lr;
+          if (lq + 4 * v == lr && i < R) {
+            const double nrm = sqrt(fabs(acc[v]));
+            cs[i] = nrm;
+            if (nrm != nrm || nrm == 0.0) atomicOr(a.err, 1);
+          }
+        }
       }
     }
   }
   __syncthreads();
-  // coef_i = ||X(i,:)||, F_i^k = X(i,:) / coef_i
-  const bool live = i < R;
-  double ss = 0.0;
-  if (live)
-    for (int c = g; c < n; c += 4) ss = fma(T[c * 64 + i], T[c * 64 + i], ss);
-  red[TID] = ss;
-  __syncthreads();
-  const double nrm = sqrt(fabs((red[i] + red[64 + i]) + (red[128 + i] + red[192 + i])));
-  if (g == 0) {
-    cs[i] = live ? nrm : 0.0;
-    if (live && (nrm != nrm || nrm == 0.0)) atomicOr(a.err, 1);
-  }
-  if (live) {
-    const double rn = nrm > 0.0 ? 1.0 / nrm : 0.0;
-    double* f = colp(a.p, k, cF + i);
-    for (int c = g; c < n; c += 4) {
-      const double v = T[c * 64 + i] * rn;
-      T[c * 64 + i] = v;
-      f[c] = v;
+  // F^k = X / coef
+#pragma unroll
+  for (int j = 0; j < NCW; j++) {
+    const int m = 16 * (g + 4 * j) + lr;
+    if (m < n) {
+#pragma unroll
+      for (int rt = 0; rt < RTM; rt++)
+        if (rt < RT) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int i = 16 * rt + lq + 4 * v;
+            if (i < R) {
+              const double nrm = cs[i];
+              const double rn = nrm > 0.0 ? 1.0 / nrm : 0.0;
+              colp(a.p, k, cF + i)[m] = x[j][rt][v] * rn;
+            }
+          }
+        }
     }
   }
-  __syncthreads();
-  // G_k = F^k F^kT on the matrix cores: 16 x 16 tiles (tj <= ti) of
-  // T^T T over the basis, tile t of the upper triangle on wave t % 4; each
-  // tile adds its share of c^T (BB o G_k) c (off-diagonal tiles twice)
+  // G_k and this tile's share of c^T (BB o G_k) c (off-diagonal tiles twice)
   double part = 0.0;
   double* Gk = a.G + (long)k * a.rstr;
-  const int RT = (R + 15) >> 4, ntile = RT * (RT + 1) / 2;
-  const int lr = i & 15, lq = i >> 4, nks = (n + 3) >> 2;
-  for (int t = g; t < ntile; t += 4) {
-    int tj = 0, rem = t;
-    while (rem >= RT - tj) { rem -= RT - tj; tj++; }
-    const int ti = tj + rem;
-    double bb[4];
-    bool ok[4];
-    const int ic = 16 * ti + lr;
+  const double* bbs = a.bbs;
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int jr = 16 * tj + lq + 4 * v;
-      ok[v] = jr < R && ic < R;
-      bb[v] = ok[v] ? bbs[ic + jr * R] : 0.0;
-    }
-    v4d acc = {0.0, 0.0, 0.0, 0.0};
-    for (int ks0 = 0; ks0 < nks; ks0 += 8) {
-      double ta[8], tb[8];
+  for (int q = 0; q < 3; q++) {
+    const int tj = gtj[q], ti = gti[q];
+    if (tj >= 0) {
+      const int ic = 16 * ti + lr;
+      const double wt = (tj == ti) ? 1.0 : 2.0;
 #pragma unroll
-      for (int b = 0; b < 8; b++) {
-        const int c = 4 * (ks0 + b) + lq;
-        ta[b] = c < n ? T[c * 64 + 16 * tj + lr] : 0.0;
-        tb[b] = c < n ? T[c * 64 + ic] : 0.0;
-      }
-#pragma unroll
-      for (int b = 0; b < 8; b++)
-        if (ks0 + b < nks) acc = mfma_f64_4(ta[b], tb[b], acc);
-    }
-    const double wt = (tj == ti) ? 1.0 : 2.0;
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int jr = 16 * tj + lq + 4 * v;
-      if (ok[v]) {
-        Gk[ic + jr * R] = acc[v];
-        if (tj != ti) Gk[jr + ic * R] = acc[v];
-        part += wt * bb[v] * acc[v] * cs[ic] * cs[jr];
+      for (int v = 0; v < 4; v++) {
+        const int jr = 16 * tj + lq + 4 * v;
+        if (jr < R && ic < R) {
+          const double ni = cs[ic], nj = cs[jr];
+          const double ri = ni > 0.0 ? 1.0 / ni : 0.0, rj = nj > 0.0 ? 1.0 / nj : 0.0;
+          const double gk = gt[q][v] * ri * rj;
+          Gk[ic + jr * R] = gk;
+          if (tj != ti) Gk[jr + ic * R] = gk;
+          part += wt * bbs[ic + jr * R] * gk * ni * nj;
+        }
       }
     }
   }
   const double tot = wg_sum(part, red);
   const double rs = 1.0 / sqrt(fabs(tot));
-  if (live && g == 0) a.p.coef[cF + i] = cs[i] * rs;
+  if (TID < R) a.p.coef[cF + TID] = cs[TID] * rs;
 }
 
 // --------------------------------------------------------------------------
@@ -1101,33 +1361,73 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_refresh(LcArgs a,
 // part(i,m) = sum_c w_c cO_c prod_{d!=k} X_d(i,c) O_c^k(m)
 __global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
                                                      const int2* jobs, int k) {
-  extern __shared__ double sm[];
+  __shared__ double P[LC_CHUNK * MLCP_RMAX];   // P(c, i), c-major
   int2 jb = jobs[blockIdx.x];
   int t = jb.x, ch = jb.y, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof;
   int c0 = ch * LC_CHUNK, nc = min(LC_CHUNK, a.ncold - c0);
-  double* P = sm;                    // nc*R
-  double* O = sm + LC_CHUNK * R;     // nc*n
   const double* w = a.wcol + (long)t * a.ncold;
   long xs = (long)a.xr * a.ncold;
   const double* X = a.X + (long)t * D * xs;
-  for (int e = TID; e < nc * R; e += MLCP_NT) {
-    int i = e % R, c = c0 + e / R;
-    double pr = w[c] * a.p.coef[a.ocol0 + c];
-    for (int m = 1; m < D; m++)
-      pr *= X[(long)((k + m) % D) * xs + (long)c * a.xr + i];
-    P[e] = pr;
+  // P(c, i) = w_c coef_c prod_{m != k} X_m(i, c): every thread owns up to
+  // LC_CHUNK * MLCP_RMAX / MLCP_NT entries, mode-outer so the loads of one
+  // mode are all in flight together
+  constexpr int PE = LC_CHUNK * MLCP_RMAX / MLCP_NT;
+  double pr[PE];
+  long off[PE];
+#pragma unroll
+  for (int j = 0; j < PE; j++) {
+    int e = TID + j * MLCP_NT;
+    bool ok = e < nc * R;
+    int i = ok ? e % R : 0, c = ok ? e / R : 0;
+    off[j] = (long)(c0 + c) * a.xr + i;
+    pr[j] = ok ? w[c0 + c] * a.p.coef[a.ocol0 + c0 + c] : 0.0;
   }
-  for (int e = TID; e < nc * n; e += MLCP_NT) {
-    int c = e / n, m = e % n;
-    O[e] = colp(a.p, k, a.ocol0 + c0 + c)[m];
+  for (int m = 1; m < D; m++) {
+    const double* Xm = X + (long)((k + m) % D) * xs;
+    double v[PE];
+#pragma unroll
+    for (int j = 0; j < PE; j++)
+      v[j] = (TID + j * MLCP_NT < nc * R) ? Xm[off[j]] : 0.0;
+#pragma unroll
+    for (int j = 0; j < PE; j++) pr[j] *= v[j];
+  }
+#pragma unroll
+  for (int j = 0; j < PE; j++) {
+    int e = TID + j * MLCP_NT;
+    if (e < nc * R) P[e] = pr[j];
   }
   __syncthreads();
+  // part(i, m) = sum_c P(c, i) O_c^k(m) on the FP64 matrix cores: tile q of
+  // the (R/16) x (n/16) output grid goes to wave q % 4; a k-step is four
+  // old columns (P rows past nc are never read: the operand is zero)
   double* part = a.part + ((long)t * a.nchunk + ch) * a.pstr;
-  for (int e = TID; e < R * n; e += MLCP_NT) {
-    int i = e / n, m = e % n;
-    double s = 0.0;
-    for (int c = 0; c < nc; c++) s += P[c * R + i] * O[c * n + m];
-    part[e] = s;
+  const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
+  const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
+  const int RT = (R + 15) >> 4, CT = (n + 15) >> 4, nks = (nc + 3) >> 2;
+  for (int q = g; q < RT * CT; q += 4) {
+    const int rt = q % RT, ct = q / RT;
+    const int i = 16 * rt + lr, mcol = 16 * ct + lr;
+    const bool iok = i < R, mok = mcol < n;
+    v4d acc = (v4d){0.0, 0.0, 0.0, 0.0};
+    for (int ks0 = 0; ks0 < nks; ks0 += 8) {
+      double av[8], bv[8];
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const int c = 4 * (ks0 + b) + lq;
+        const bool cok = c < nc;
+        av[b] = (cok && iok) ? P[c * R + i] : 0.0;
+        bv[b] = (cok && mok) ? colp(a.p, k, a.ocol0 + c0 + c)[mcol] : 0.0;
+      }
+#pragma unroll
+      for (int b = 0; b < 8; b++)
+        if (ks0 + b < nks) acc = mfma_f64_4(av[b], bv[b], acc);
+    }
+    // element v of lane l: row 16rt + l/16 + 4v, column 16ct + l%16
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int io = 16 * rt + lq + 4 * v;
+      if (io < R && mok) part[io * n + mcol] = acc[v];
+    }
   }
 }
 
@@ -1139,11 +1439,19 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
   int t = blockIdx.x, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof, xs = n | 1;
   double* X = sm;
   double* A = sm + R * xs;
+  // sum of the chunk partials, eight independent chains so the loads of
+  // eight chunks are in flight together
+  const double* pt = a.part + (long)t * a.nchunk * a.pstr;
   for (int e = TID; e < R * n; e += MLCP_NT) {
-    double s = 0.0;
-    for (int ch = 0; ch < a.nchunk; ch++)
-      s += a.part[((long)t * a.nchunk + ch) * a.pstr + e];
-    X[(e / n) * xs + e % n] = s;
+    double s8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int ch = 0;
+    for (; ch + 8 <= a.nchunk; ch += 8) {
+#pragma unroll
+      for (int u = 0; u < 8; u++) s8[u] += pt[(long)(ch + u) * a.pstr + e];
+    }
+    for (int u = 0; ch < a.nchunk; ch++, u++) s8[u] += pt[(long)ch * a.pstr + e];
+    X[(e / n) * xs + e % n] =
+        ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
   }
   const double* G = a.G + (long)t * D * a.rstr;
   double pen = a.pen[t];

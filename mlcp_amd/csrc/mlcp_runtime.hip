@@ -77,9 +77,8 @@ static int nx_of(int nmax) {
   return need <= 8 ? 8 : need <= 12 ? 12 : need <= 16 ? 16 : need <= 24 ? 24 : 32;
 }
 
-// Dynamic LDS limit per workgroup: the device's per-block maximum less a
-// margin for the kernels' static LDS.
-static size_t g_lds_limit = 64 * 1024;
+// LDS per workgroup of the device (static plus dynamic).
+static size_t g_lds_dev = 64 * 1024;
 
 static int setup_kernels() {
   static bool done = false;
@@ -91,31 +90,42 @@ static int setup_kernels() {
 #define NX_KERNELS(kern)                                                    \
   (const void*)kern<8>, (const void*)kern<12>, (const void*)kern<16>,       \
       (const void*)kern<24>, (const void*)kern<32>
-  const void* ks[] = {(const void*)k_op_refresh,   (const void*)k_lc_rhs,
-                      (const void*)k_op_hadamard,  NX_KERNELS(k_gs_terms),
-                      NX_KERNELS(k_op_solve),      NX_KERNELS(k_gs_solve),
+#define NX_KERNELS2(kern, m)                                                \
+  (const void*)kern<8, m>, (const void*)kern<12, m>,                        \
+      (const void*)kern<16, m>, (const void*)kern<24, m>,                   \
+      (const void*)kern<32, m>
+  const void* ks[] = {(const void*)k_op_refresh,   (const void*)k_op_hadamard,
+                      NX_KERNELS(k_gs_terms),      NX_KERNELS(k_op_solve),
+                      NX_KERNELS2(k_gs_solve, 2),  NX_KERNELS2(k_gs_solve, 4),
                       NX_KERNELS(k_lc_solve)};
+#undef NX_KERNELS2
 #undef NX_KERNELS
-  size_t lim = (size_t)v;
+  g_lds_dev = (size_t)v;
   for (const void* k : ks) {
     hipFuncAttributes fa;
     HCHK(hipFuncGetAttributes(&fa, k));
     size_t kl = (size_t)v - fa.sharedSizeBytes;
     HCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)kl));
-    lim = std::min(lim, kl);
   }
-  g_lds_limit = lim;
   done = true;
   return 0;
 }
 
+// Dynamic LDS a kernel may request: the device's per-block maximum less the
+// kernel's static LDS.
+static size_t lds_limit(const void* kern) {
+  hipFuncAttributes fa;
+  if (hipFuncGetAttributes(&fa, kern) != hipSuccess) return 0;
+  return g_lds_dev > fa.sharedSizeBytes ? g_lds_dev - fa.sharedSizeBytes : 0;
+}
+
 // Host-side check of a launch's dynamic LDS request.
-static int lds_ok(size_t bytes, const char* kernel) {
-  if (bytes > g_lds_limit)
+static int lds_ok(size_t bytes, const char* kernel, const void* kern) {
+  const size_t lim = lds_limit(kern);
+  if (bytes > lim)
     return fail(std::string(kernel) + ": needs " + std::to_string(bytes) +
-                " bytes of LDS, the device allows " +
-                std::to_string(g_lds_limit));
+                " bytes of LDS, the device allows " + std::to_string(lim));
   return 0;
 }
 
@@ -156,7 +166,7 @@ struct mlcp_session {
   std::vector<int> njobs_mode;
   int njobs_all = 0, njobs_had = 0;
   // Gram-Schmidt
-  DBuf<double> gsX, gsG, gsPart, gsBbs, gsPc, gsBjk;
+  DBuf<double> gsX, gsG, gsPart, gsBbs, gsPc, gsBjk, gsGjf;
   int had_tile = MLCP_NT;
   // linear-combination ALS
   DBuf<double> lcX, lcG, lcPart, lcW;
@@ -412,9 +422,10 @@ static int session_alloc(mlcp_session* s) {
   // of the tile (and the entry holding 1) fits in LDS
   {
     size_t per = (size_t)(s->nops + 1) * sizeof(double);
+    const size_t lim = lds_limit((const void*)k_op_hadamard);
     int E = MLCP_NT;
-    while (E > 64 && per * E > g_lds_limit) E -= 64;
-    if (per * E > g_lds_limit)
+    while (E > 64 && per * E > lim) E -= 64;
+    if (per * E > lim)
       return fail("too many operators (" + std::to_string(s->nops) +
                   ") for the LDS of one workgroup");
     s->had_tile = E;
@@ -430,7 +441,8 @@ static int session_alloc(mlcp_session* s) {
   int pstr = s->Rmax * s->nmax;
   if (s->gsX.need((size_t)B * D * rstr) || s->gsG.need((size_t)D * rstr) ||
       s->gsPart.need((size_t)(B + 1) * pstr) || s->gsBbs.need(rstr) ||
-      s->gsPc.need((size_t)(B + 1) * GS_PCS) || s->gsBjk.need(pstr))
+      s->gsPc.need((size_t)(B + 1) * GS_PCS) || s->gsBjk.need(pstr) ||
+      s->gsGjf.need(GJF_SIZE))
     return -1;
   // GEP
   std::vector<PairJob> pj;
@@ -457,12 +469,25 @@ static int check_lds(int Rmax, int nmax, int nbloc) {
   size_t r = Rmax, n = nmax, b = sizeof(double);
   size_t n1 = n | 1;
   (void)nbloc;
-  if (lds_ok(2 * n * r * b, "k_op_refresh") ||
-      lds_ok((r * n1 + r * r + OPS_JCH * r) * b, "k_op_solve") ||
-      lds_ok((r * n1 + r * r) * b, "k_lc_solve") ||
-      lds_ok(64 * n * b, "k_gs_solve") ||
-      lds_ok(256 * (size_t)nx_of((int)n) * b, "k_gs_terms") ||
-      lds_ok(((size_t)LC_CHUNK * r + (size_t)LC_CHUNK * n) * b, "k_lc_rhs"))
+  const int nx = nx_of((int)n);
+#define NX_PTR(kern)                                                        \
+  (nx == 8 ? (const void*)kern<8> : nx == 12 ? (const void*)kern<12>        \
+   : nx == 16 ? (const void*)kern<16> : nx == 24 ? (const void*)kern<24>    \
+              : (const void*)kern<32>)
+#define NX_PTR2(kern, m)                                                    \
+  (nx == 8 ? (const void*)kern<8, m> : nx == 12 ? (const void*)kern<12, m>  \
+   : nx == 16 ? (const void*)kern<16, m>                                    \
+   : nx == 24 ? (const void*)kern<24, m> : (const void*)kern<32, m>)
+  const bool bad =
+      lds_ok(2 * n * r * b, "k_op_refresh", (const void*)k_op_refresh) ||
+      lds_ok((r * n1 + r * r + OPS_JCH * r) * b, "k_op_solve",
+             NX_PTR(k_op_solve)) ||
+      lds_ok((r * n1 + r * r) * b, "k_lc_solve", NX_PTR(k_lc_solve)) ||
+      lds_ok(65 * n * b, "k_gs_solve", NX_PTR2(k_gs_solve, 4)) ||
+      lds_ok(256 * (size_t)nx * b, "k_gs_terms", NX_PTR(k_gs_terms));
+#undef NX_PTR
+#undef NX_PTR2
+  if (bad)
     return -1;
   return 0;
 }
@@ -547,6 +572,27 @@ static int check_err(mlcp_session* s, const char* phase) {
         break;                                                            \
       default:                                                            \
         PLAUNCH(kid, (kern<32>), grid, dim3(MLCP_NT), lds, __VA_ARGS__);  \
+    }                                                                     \
+  } while (0)
+
+// LAUNCH_NX for kernels with a second template parameter m
+#define LAUNCH_NX2(kid, kern, m, grid, lds, ...)                            \
+  do {                                                                    \
+    switch (nx_of(s->nmax)) {                                             \
+      case 8:                                                             \
+        PLAUNCH(kid, (kern<8, m>), grid, dim3(MLCP_NT), lds, __VA_ARGS__);  \
+        break;                                                            \
+      case 12:                                                            \
+        PLAUNCH(kid, (kern<12, m>), grid, dim3(MLCP_NT), lds, __VA_ARGS__); \
+        break;                                                            \
+      case 16:                                                            \
+        PLAUNCH(kid, (kern<16, m>), grid, dim3(MLCP_NT), lds, __VA_ARGS__); \
+        break;                                                            \
+      case 24:                                                            \
+        PLAUNCH(kid, (kern<24, m>), grid, dim3(MLCP_NT), lds, __VA_ARGS__); \
+        break;                                                            \
+      default:                                                            \
+        PLAUNCH(kid, (kern<32, m>), grid, dim3(MLCP_NT), lds, __VA_ARGS__); \
     }                                                                     \
   } while (0)
 
@@ -689,6 +735,7 @@ static GsArgs gs_args(mlcp_session* s, int iv) {
   g.bbs = s->gsBbs.p;
   g.pc = s->gsPc.p;
   g.bjk = s->gsBjk.p;
+  g.gjf = s->gsGjf.p;
   g.err = s->err.p;
   return g;
 }
@@ -745,11 +792,19 @@ static int launch_ortho(mlcp_session* s, int nals) {
       for (int k = 0; k < D; k++) {
         LAUNCH_NX(KID_GS_TERMS, k_gs_terms, dim3((iv + 1) * nch * g.xst),
                   lds_t, g, k, kprev);
-        int nred = (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
-        hipLaunchKernelGGL(k_gs_reduce, dim3(nred * g.xst), dim3(MLCP_NT), 0,
-                           s->st, g, k);
-        size_t lds = (size_t)64 * s->nbas[k] * sizeof(double);
-        LAUNCH_NX(KID_GS_SOLVE, k_gs_solve, dim3(1), lds, g, k);
+        // workgroup 0 factors the system matrix, the others sum the
+        // right-hand side; the solve applies the factors
+        int nred = 1 + (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
+        size_t lds = (size_t)65 * s->nbas[k] * sizeof(double);
+        if (s->ranks[iv] <= 32) {
+          hipLaunchKernelGGL(k_gs_reduce<2>, dim3(nred * g.xst), dim3(MLCP_NT),
+                             0, s->st, g, k);
+          LAUNCH_NX2(KID_GS_SOLVE, k_gs_solve, 2, dim3(1), lds, g, k);
+        } else {
+          hipLaunchKernelGGL(k_gs_reduce<4>, dim3(nred * g.xst), dim3(MLCP_NT),
+                             0, s->st, g, k);
+          LAUNCH_NX2(KID_GS_SOLVE, k_gs_solve, 4, dim3(1), lds, g, k);
+        }
         kprev = k;
       }
   }
@@ -808,9 +863,7 @@ static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
   }
   for (int it = 0; it < nitn; it++)
     for (int k = 0; k < D; k++) {
-      size_t lds_r = ((size_t)LC_CHUNK * s->Rmax +
-                      (size_t)LC_CHUNK * s->nbas[k]) * sizeof(double);
-      PLAUNCH(KID_LC_RHS, k_lc_rhs, dim3(nj), dim3(MLCP_NT), lds_r, a,
+      PLAUNCH(KID_LC_RHS, k_lc_rhs, dim3(nj), dim3(MLCP_NT), 0, a,
               (const int2*)s->lcJobs.p, k);
       size_t lds_s = ((size_t)s->Rmax * (s->nbas[k] | 1) +
                       (size_t)s->Rmax * s->Rmax) * sizeof(double);
