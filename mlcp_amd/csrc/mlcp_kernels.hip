@@ -320,47 +320,49 @@ __device__ __forceinline__ double sel4(double a0, double a1, double a2,
   return __longlong_as_double((lo & ~m1) | (hi & m1));
 }
 
-// Gauss-Jordan factors of the SPD R x R matrix A(r, c) + pen delta_rc
-// (R <= 16 RTM <= 64), computed by wave 0 alone (no workgroup barrier; the
-// other waves return).  Step u (pivot rows/columns J = 4u..4u+3, pivot block P) is
-//   W <- W + E_u W(J, :),  E_u(i, :) = -W(i, J) P^-1 (i not in J),
-//                          E_u(J_r, :) = P^-1(r, :) - e_r,
-// i.e. rows J become P^-1 W(J, :) and the others lose W(i, J) P^-1 W(J, :).
-// P^-1 comes from inv4_spd.  The matrix is held as 16 x 16 tiles in the
-// MFMA result layout (element v of lane l: row 16rt + l/16 + 4v, column
-// 16ct + l%16); E_u for row tile rt is stored in the A-operand layout
-// (lane l: row 16rt + l%16, column l/16) at f[(u*4 + rt)*64 + lane], and
-// W(J, ct-tile) is the B operand of the update held by the same lane as
-// element u%4 of tile (u/4, ct), so applying the steps to right-hand sides
-// needs no lane exchange.  f[16*256] = 1 if a pivot is not positive, else
-// 0.  sc: GJF_SC doubles of LDS.
 #define GJF_SC 256
-template <int RTM, typename FA>
-__device__ __forceinline__ void gj_factor(FA A, double pen, int R, double* f,
-                                          double* sc) {
-  if (TID >= 64) return;
-  const int lane = TID, lr = lane & 15, lq = lane >> 4;
-  const int RT = (R + 15) >> 4, nst = (R + 3) >> 2;
-  const v4d zero = {0.0, 0.0, 0.0, 0.0};
-  // tiles w[rt][ct] (RT <= RTM); every lane loads a clamped (finite) entry,
-  // selected through an FMA, so all loads are in flight together
-  v4d w[RTM][RTM];
+
+// Matrix tiles w[rt][ct] of gj_factor (lanes of wave 0): A(i, c) = BB(i, c)
+// + pen delta_ic from the column-major BB, identity beyond R.  Every lane
+// loads a clamped (finite) entry and selects it through an FMA, so all loads
+// are in flight together.
+template <int RTM>
+__device__ __forceinline__ void gjf_load_bb(v4d (&w)[RTM][RTM], const double* bb,
+                                            double pen, int R) {
+  const int lr = TID & 15, lq = (TID >> 4) & 3, RT = (R + 15) >> 4;
 #pragma unroll
   for (int rt = 0; rt < RTM; rt++)
 #pragma unroll
-    for (int ct = 0; ct < RTM; ct++) {
-      w[rt][ct] = zero;
-      if (rt < RT && ct < RT) {   // uniform
+    for (int ct = 0; ct < RTM; ct++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int i = 16 * rt + lq + 4 * v, c = 16 * ct + lr;
-          const double a = A(clampi(i, R - 1), clampi(c, R - 1));
-          const bool in = i < R && c < R;
-          const double idv = (!in && i == c) ? 1.0 : 0.0;
-          w[rt][ct][v] = fma(in ? 1.0 : 0.0, a + (i == c ? pen : 0.0), idv);
-        }
+      for (int v = 0; v < 4; v++) {
+        const int i = 16 * rt + lq + 4 * v, c = 16 * ct + lr;
+        const bool in = rt < RT && ct < RT && i < R && c < R;
+        const double a = bb[clampi(i, R - 1) + clampi(c, R - 1) * R];
+        const double idv = (!in && i == c) ? 1.0 : 0.0;
+        w[rt][ct][v] = fma(in ? 1.0 : 0.0, a + (i == c ? pen : 0.0), idv);
       }
-    }
+}
+
+// Gauss-Jordan factors of the SPD R x R matrix held in w (gjf_load_bb;
+// R <= 16 RTM <= 64), computed by one wave (no workgroup barrier).  Step u
+// (pivot rows/columns J = 4u..4u+3, pivot block P) is
+//   W <- W + E_u W(J, :),  E_u(i, :) = -W(i, J) P^-1 (i not in J),
+//                          E_u(J_r, :) = P^-1(r, :) - e_r,
+// i.e. rows J become P^-1 W(J, :) and the others lose W(i, J) P^-1 W(J, :).
+// P^-1 comes from inv4_spd.  The tiles are in the MFMA result layout
+// (element v of lane l: row 16rt + l/16 + 4v, column 16ct + l%16); E_u for
+// row tile rt is stored in the A-operand layout (lane l: row 16rt + l%16,
+// column l/16) at f[(u*4 + rt)*64 + lane], and W(J, ct-tile) is the B
+// operand of the update held by the same lane as element u%4 of tile
+// (u/4, ct), so applying the steps to right-hand sides needs no lane
+// exchange.  f[16*256] = 1 if a pivot is not positive, else 0.  sc: GJF_SC
+// doubles of LDS.  Call with the 64 lanes of one wave.
+template <int RTM>
+__device__ __forceinline__ void gj_factor(v4d (&w)[RTM][RTM], int R, double* f,
+                                          double* sc) {
+  const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
+  const int RT = (R + 15) >> 4, nst = (R + 3) >> 2;
   double* colB = sc;   // [c*64 + row]: the four pivot columns
   bool bad = false;
 #pragma unroll
@@ -902,6 +904,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
   const int job = blockIdx.x / a.xst;
   const int iv = a.iv, D = a.p.ndof;
   const int R = a.Riv, cF = a.cFiv, n = a.p.nbas[k];
+
   const int nch = (R + GS_ICH - 1) / GS_ICH;
   const int l = job / nch, ic = job % nch;
   const int i0 = ic * GS_ICH, ni = min(GS_ICH, R - i0);
@@ -1037,8 +1040,8 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
 // same XCD as k_gs_terms): coef_l = -sum of the partial sums and
 // bjk = sum_{l<iv} coef_l P_l + P_iv (previous states first, ALSUtils.f90
 // order), for a slice of MLCP_NT elements per workgroup.  Workgroup 0
-// instead factors the step's system matrix BB + pen I (gj_factor, rank
-// R <= 16 RTM), so k_gs_solve only applies the elimination.
+// instead factors the step's system matrix for k_gs_solve (gj_factor on one
+// wave), which leaves the solve only the elimination of the right-hand side.
 template <int RTM>
 __global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k) {
   __shared__ double cl[MLCP_BMAX];
@@ -1047,10 +1050,12 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k) {
   int job = blockIdx.x / a.xst;
   int iv = a.iv, R = a.Riv, n = a.p.nbas[k];
   if (job == 0) {
-    // factors of the step's system matrix BB + pen I for k_gs_solve
-    const double* bb = a.bbs;
-    gj_factor<RTM>([&](int r, int c) { return bb[r + c * R]; }, *a.pen, R,
-                   a.gjf, fsc);
+    // workgroup 0: factors of the step's system matrix BB + pen I (one wave)
+    if (TID < 64) {
+      v4d w[RTM][RTM];
+      gjf_load_bb<RTM>(w, a.bbs, *a.pen, R);
+      gj_factor<RTM>(w, R, a.gjf, fsc);
+    }
     return;
   }
   job -= 1;
@@ -1109,7 +1114,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
   constexpr int NCW = (NX / 4 + 3) / 4;   // column tiles of B per wave
   extern __shared__ double T[];
   __shared__ double red[8];
-  __shared__ double cs[64];
+  __shared__ double cs[64], rcs[64];
   const int R = a.Riv, cF = a.cFiv, n = a.p.nbas[k];
   const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
   const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
@@ -1144,6 +1149,25 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
       e[u][rt] = 0.0;
       if (u < nst && rt < RT) e[u][rt] = f[(u * 4 + rt) * 64 + lane];
     }
+  // Gram tiles of this wave (tile t = g + 4q of the upper triangle,
+  // tj <= ti) and their BB entries, loaded with the rest
+  const int ntile = RT * (RT + 1) / 2;
+  int gtj[3], gti[3];
+  double bbv[3][4];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const int t = g + 4 * q;
+    int tj = 0, rem = t;
+    while (tj < RT && rem >= RT - tj) { rem -= RT - tj; tj++; }
+    gtj[q] = t < ntile ? tj : -1;
+    gti[q] = t < ntile ? tj + rem : -1;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int ic = 16 * gti[q] + lr, jr = 16 * gtj[q] + lq + 4 * v;
+      bbv[q][v] = ld_or0(a.bbs + clampi(ic, R - 1) + clampi(jr, R - 1) * R,
+                         t < ntile && ic < R && jr < R);
+    }
+  }
   if (f[16 * 256] != 0.0) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
@@ -1178,23 +1202,15 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
   }
   __syncthreads();
   // Gram G = X X^T on the matrix cores, tile t of the upper triangle on
-  // wave t % 4 (at most 10 tiles: three per wave)
-  const int ntile = RT * (RT + 1) / 2, nks = (n + 3) >> 2;
+  // wave t % 4 (at most 10 tiles: three per wave), two accumulators per tile
+  const int nks = (n + 3) >> 2;
   v4d gt[3];
-  int gtj[3], gti[3];
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    const int t = g + 4 * q;
     gt[q] = zero;
-    gtj[q] = -1;
-    gti[q] = -1;
-    if (t < ntile) {
-      int tj = 0, rem = t;
-      while (rem >= RT - tj) { rem -= RT - tj; tj++; }
-      const int ti = tj + rem;
-      gtj[q] = tj;
-      gti[q] = ti;
-      v4d acc = zero;
+    const int tj = gtj[q], ti = gti[q];
+    if (tj >= 0) {
+      v4d acc[2] = {zero, zero};
       for (int ks0 = 0; ks0 < nks; ks0 += 8) {
         double ta[8], tb[8];
 #pragma unroll
@@ -1205,17 +1221,18 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
         }
 #pragma unroll
         for (int b = 0; b < 8; b++)
-          if (ks0 + b < nks) acc = mfma_f64_4(ta[b], tb[b], acc);
+          if (ks0 + b < nks) acc[b & 1] = mfma_f64_4(ta[b], tb[b], acc[b & 1]);
       }
-      gt[q] = acc;
+      gt[q] = acc[0] + acc[1];
       // element v: row j = 16tj + l/16 + 4v, column i = 16ti + l%16
       if (tj == ti) {
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const int i = 16 * ti + lr;
           if (lq + 4 * v == lr && i < R) {
-            const double nrm = sqrt(fabs(acc[v]));
+            const double nrm = sqrt(fabs(gt[q][v]));
             cs[i] = nrm;
+            rcs[i] = nrm > 0.0 ? 1.0 / nrm : 0.0;
             if (nrm != nrm || nrm == 0.0) atomicOr(a.err, 1);
           }
         }
@@ -1234,11 +1251,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
 #pragma unroll
           for (int v = 0; v < 4; v++) {
             const int i = 16 * rt + lq + 4 * v;
-            if (i < R) {
-              const double nrm = cs[i];
-              const double rn = nrm > 0.0 ? 1.0 / nrm : 0.0;
-              colp(a.p, k, cF + i)[m] = x[j][rt][v] * rn;
-            }
+            if (i < R) colp(a.p, k, cF + i)[m] = x[j][rt][v] * rcs[i];
           }
         }
     }
@@ -1246,7 +1259,6 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
   // G_k and this tile's share of c^T (BB o G_k) c (off-diagonal tiles twice)
   double part = 0.0;
   double* Gk = a.G + (long)k * a.rstr;
-  const double* bbs = a.bbs;
 #pragma unroll
   for (int q = 0; q < 3; q++) {
     const int tj = gtj[q], ti = gti[q];
@@ -1257,12 +1269,10 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
       for (int v = 0; v < 4; v++) {
         const int jr = 16 * tj + lq + 4 * v;
         if (jr < R && ic < R) {
-          const double ni = cs[ic], nj = cs[jr];
-          const double ri = ni > 0.0 ? 1.0 / ni : 0.0, rj = nj > 0.0 ? 1.0 / nj : 0.0;
-          const double gk = gt[q][v] * ri * rj;
+          const double gk = gt[q][v] * rcs[ic] * rcs[jr];
           Gk[ic + jr * R] = gk;
           if (tj != ti) Gk[jr + ic * R] = gk;
-          part += wt * bbs[ic + jr * R] * gk * ni * nj;
+          part += wt * bbv[q][v] * gk * cs[ic] * cs[jr];
         }
       }
     }

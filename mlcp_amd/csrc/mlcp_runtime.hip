@@ -792,10 +792,10 @@ static int launch_ortho(mlcp_session* s, int nals) {
       for (int k = 0; k < D; k++) {
         LAUNCH_NX(KID_GS_TERMS, k_gs_terms, dim3((iv + 1) * nch * g.xst),
                   lds_t, g, k, kprev);
-        // workgroup 0 factors the system matrix, the others sum the
-        // right-hand side; the solve applies the factors
-        int nred = 1 + (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
-        size_t lds = (size_t)65 * s->nbas[k] * sizeof(double);
+        // workgroup 0 of k_gs_reduce factors the system matrix, the others
+        // sum the right-hand side; the solve applies the factors
+        const int nred = 1 + (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
+        const size_t lds = (size_t)65 * s->nbas[k] * sizeof(double);
         if (s->ranks[iv] <= 32) {
           hipLaunchKernelGGL(k_gs_reduce<2>, dim3(nred * g.xst), dim3(MLCP_NT),
                              0, s->st, g, k);
