@@ -1506,25 +1506,55 @@ struct PairJob {
   long out;
 };
 
+// The R_A x R_B matrix of per-mode inner products is formed on the FP64
+// matrix cores: 16 x 16 tile t goes to wave t % 4, and for every mode the
+// tile of <A_p^d, B_q^d> (MFMA k-steps of 4 basis functions) is multiplied
+// into the running product elementwise -- the result layout is the same for
+// every mode.  The weighted sum sum cA_p cB_q prod_d(...) ends in wg_sum.
 __global__ __launch_bounds__(MLCP_NT) void k_pair_inner(Pool p,
                                                          const PairJob* jobs,
                                                          double* out) {
   __shared__ double red[8];
-  PairJob jb = jobs[blockIdx.x];
+  const PairJob jb = jobs[blockIdx.x];
+  const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
+  const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
+  const int ra = jb.ra, rb = jb.rb;
+  const int RTa = (ra + 15) >> 4, RTb = (rb + 15) >> 4, nt = RTa * RTb;
   double acc = 0.0;
-  for (int e = TID; e < jb.ra * jb.rb; e += MLCP_NT) {
-    int i = e / jb.rb, j = e % jb.rb;
-    double g = 1.0;
+  for (int t = g; t < nt; t += 4) {
+    const int ta = t / RTb, tb = t % RTb;
+    const int ia = 16 * ta + lr, ib = 16 * tb + lr;
+    const bool oka = ia < ra, okb = ib < rb;
+    v4d prod = {1.0, 1.0, 1.0, 1.0};
     for (int d = 0; d < p.ndof; d++) {
-      const double* x = colp(p, d, jb.ca + i);
-      const double* y = colp(p, d, jb.cb + j);
-      double dt = 0.0;
-      for (int m = 0; m < p.nbas[d]; m++) dt += x[m] * y[m];
-      g *= dt;
+      const int n = p.nbas[d], nks = (n + 3) >> 2;
+      const double* x = colp(p, d, jb.ca + (oka ? ia : 0));
+      const double* y = colp(p, d, jb.cb + (okb ? ib : 0));
+      v4d c = {0.0, 0.0, 0.0, 0.0};
+      for (int ks0 = 0; ks0 < nks; ks0 += 8) {
+        double av[8], bv[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+          const int m = 4 * (ks0 + b) + lq;
+          const int mc = min(m, n - 1);
+          av[b] = ld_or0(x + mc, oka && m < n);
+          bv[b] = ld_or0(y + mc, okb && m < n);
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+          if (ks0 + b < nks) c = mfma_f64_4(av[b], bv[b], c);
+      }
+      prod = prod * c;
     }
-    acc += p.coef[jb.ca + i] * p.coef[jb.cb + j] * g;
+    // element v: p = 16ta + l/16 + 4v, q = 16tb + l%16
+    const double cb = okb ? p.coef[jb.cb + ib] : 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int pa = 16 * ta + lq + 4 * v;
+      if (pa < ra) acc += p.coef[jb.ca + pa] * cb * prod[v];
+    }
   }
-  double tot = wg_sum(acc, red);
+  const double tot = wg_sum(acc, red);
   if (TID == 0) out[jb.out] = tot;
 }
 
@@ -1534,6 +1564,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_pair_inner(Pool p,
 // parallel (round-robin ordered) two-sided Jacobi, ascending sort,
 // X = U^-1 V, sign: largest-|x| entry positive.  Work arrays in global memory
 // (ws: 4*n*n + 2*n doubles).  Output X column major n x n into Xo, w into wo.
+#define GEP_U 4
 __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
                                               const double* Sin, int n,
                                               double* ws, double* Xo,
@@ -1664,30 +1695,62 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
         rs[q] = s;
       }
       __syncthreads();
-      // columns of C and V
-      for (int e = tid; e < np * n; e += T) {
-        int q = e / n, r = e % n;
-        int p0 = pairs[2 * q], p1 = pairs[2 * q + 1];
-        if (p1 >= n || rs[q] == 0.0) continue;
-        double c = rc[q], s = rs[q];
-        double x = C[r + (long)p0 * n], y = C[r + (long)p1 * n];
-        C[r + (long)p0 * n] = c * x - s * y;
-        C[r + (long)p1 * n] = s * x + c * y;
-        x = V[r + (long)p0 * n];
-        y = V[r + (long)p1 * n];
-        V[r + (long)p0 * n] = c * x - s * y;
-        V[r + (long)p1 * n] = s * x + c * y;
+      // columns of C and V, then rows of C.  Every thread first loads all
+      // of its (up to GEP_U per pass) element pairs -- from offset 0 when the
+      // pair has no rotation -- and only then stores, so its loads are in
+      // flight together instead of one round trip per element.
+      for (int e0 = 0; e0 < np * n; e0 += GEP_U * T) {
+        double cx[GEP_U], cy[GEP_U], vx[GEP_U], vy[GEP_U], cc[GEP_U], ss[GEP_U];
+        long o0[GEP_U], o1[GEP_U];
+        bool live[GEP_U];
+#pragma unroll
+        for (int u = 0; u < GEP_U; u++) {
+          const int e = e0 + u * T + tid;
+          const int q = e < np * n ? e / n : 0, r = e % n;
+          const int p0 = pairs[2 * q], p1 = pairs[2 * q + 1];
+          live[u] = e < np * n && p1 < n && rs[q] != 0.0;
+          o0[u] = live[u] ? r + (long)p0 * n : 0;
+          o1[u] = live[u] ? r + (long)p1 * n : 0;
+          cc[u] = rc[q];
+          ss[u] = rs[q];
+          cx[u] = C[o0[u]];
+          cy[u] = C[o1[u]];
+          vx[u] = V[o0[u]];
+          vy[u] = V[o1[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < GEP_U; u++)
+          if (live[u]) {
+            C[o0[u]] = cc[u] * cx[u] - ss[u] * cy[u];
+            C[o1[u]] = ss[u] * cx[u] + cc[u] * cy[u];
+            V[o0[u]] = cc[u] * vx[u] - ss[u] * vy[u];
+            V[o1[u]] = ss[u] * vx[u] + cc[u] * vy[u];
+          }
       }
       __syncthreads();
-      // rows of C
-      for (int e = tid; e < np * n; e += T) {
-        int q = e / n, r = e % n;
-        int p0 = pairs[2 * q], p1 = pairs[2 * q + 1];
-        if (p1 >= n || rs[q] == 0.0) continue;
-        double c = rc[q], s = rs[q];
-        double x = C[p0 + (long)r * n], y = C[p1 + (long)r * n];
-        C[p0 + (long)r * n] = c * x - s * y;
-        C[p1 + (long)r * n] = s * x + c * y;
+      for (int e0 = 0; e0 < np * n; e0 += GEP_U * T) {
+        double cx[GEP_U], cy[GEP_U], cc[GEP_U], ss[GEP_U];
+        long o0[GEP_U], o1[GEP_U];
+        bool live[GEP_U];
+#pragma unroll
+        for (int u = 0; u < GEP_U; u++) {
+          const int e = e0 + u * T + tid;
+          const int q = e < np * n ? e / n : 0, r = e % n;
+          const int p0 = pairs[2 * q], p1 = pairs[2 * q + 1];
+          live[u] = e < np * n && p1 < n && rs[q] != 0.0;
+          o0[u] = live[u] ? p0 + (long)r * n : 0;
+          o1[u] = live[u] ? p1 + (long)r * n : 0;
+          cc[u] = rc[q];
+          ss[u] = rs[q];
+          cx[u] = C[o0[u]];
+          cy[u] = C[o1[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < GEP_U; u++)
+          if (live[u]) {
+            C[o0[u]] = cc[u] * cx[u] - ss[u] * cy[u];
+            C[o1[u]] = ss[u] * cx[u] + cc[u] * cy[u];
+          }
       }
       __syncthreads();
       for (int q = tid; q < np; q += T) {
