@@ -99,6 +99,23 @@ int mlcp_hip_session_probe(mlcp_session *s, int kernel_id, int max_launches);
 int mlcp_hip_session_probe_read(mlcp_session *s, double *total_ms,
                                 long *launches);
 
+/* Launch parameters of a session, the autotuner's knobs (the launch-shape
+ * choices of PVVkernel::initnew, modvecvec.cu:27-94, made per problem and
+ * persisted instead of recomputed per call):
+ *   had_tile   elements of R x R per Hadamard workgroup (multiple of 64, at
+ *              most the LDS bound, reported as out4[3] by tune_get)
+ *   had_split  slot ranges per Hadamard job (1..8; more, shorter jobs)
+ *   gs_stride  Gram-Schmidt workgroup stride (1: all CUs, 8: one XCD)
+ * 0 keeps a value.  Results are bitwise independent of all three.  A session
+ * starts from the table $MLCP_HIP_TUNING (or tuning_table.txt beside the
+ * library; lines "<key> <had_tile> <had_split> <gs_stride>") when it holds
+ * the session's key, else from the heuristic defaults.  tune_key writes the
+ * key "<arch>/D<ndof>/n<n_1>.<n_2>.../B<nbloc>/R<min>.<max>.<sum>/o<nops>/t<T>". */
+int mlcp_hip_session_tune_set(mlcp_session *s, int had_tile, int had_split,
+                              int gs_stride);
+int mlcp_hip_session_tune_get(mlcp_session *s, int *out4);
+int mlcp_hip_session_tune_key(mlcp_session *s, char *buf, int len);
+
 /* Single phases on the resident block, for parity tests.
  *  power:    npow ALS_POW3 sweeps on every state
  *  ortho:    ALS_ORTHO_3 with nals sweeps

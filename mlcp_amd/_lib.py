@@ -92,6 +92,12 @@ def lib():
     L.mlcp_hip_update_opers.restype = _i
     L.mlcp_hip_update_opers.argtypes = [_dp, _dp, _ip, _i, _i, _ip, _i, _ip,
                                         _ip, _dp, _i, _dp]
+    L.mlcp_hip_session_tune_set.restype = _i
+    L.mlcp_hip_session_tune_set.argtypes = [_v, _i, _i, _i]
+    L.mlcp_hip_session_tune_get.restype = _i
+    L.mlcp_hip_session_tune_get.argtypes = [_v, _ip]
+    L.mlcp_hip_session_tune_key.restype = _i
+    L.mlcp_hip_session_tune_key.argtypes = [_v, C.c_char_p, _i]
     _lib = L
     return L
 

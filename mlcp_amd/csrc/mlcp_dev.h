@@ -62,6 +62,8 @@ struct OpArgs {
   // once, each record its (<= 4) other operators
   int idop[MLCP_MAXD];   // identity operator of each mode (-1: none)
   const int* hs_off;     // [ndof+1] into hs_beg/hs_end (one entry per slot)
+  const int* hsp;        // [ndof][nsplit+1] slot ranges of the Hadamard jobs
+  int nsplit;            // slot ranges per (state, tile)
   const int* hs_beg;     // [slot] first group
   const int* hs_end;
   const unsigned* hg_mask;  // [group] modes whose identity Gram is a factor

@@ -584,6 +584,11 @@ __global__ void k_penalty(Pool p, const int* col, const int* R, double scale,
 // --------------------------------------------------------------------------
 // operator ALS
 
+// Pointer to read-only global memory in the constant address space: uniform
+// loads through it are scalar loads (reads only).
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+
 // Refresh caches.  job = (s, op): op >= 0 -> M (and Y if do_y);
 // op = -(d+1) -> G_d.
 __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
@@ -644,9 +649,12 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
 }
 
 // W_u = sum_{t: op(t,k)=u} coef_t prod_{d!=k} M_{op(t,d)}  and
-// BB = prod_{d!=k} G_d.  job = (s, tile of blockDim.x elements of R x R).
-// Every cached Gram of the element is staged once in LDS (Mc[op][thread])
-// so that the term loop reads LDS instead of re-reading L2.  The terms of a
+// BB = prod_{d!=k} G_d.  job = (s, tile of E elements of R x R); a
+// workgroup has E/64 element groups of one wave per part, a part being a
+// range of the mode's slots (the ranges split the mode's records evenly).
+// Every cached Gram of the tile is staged once in LDS (Mc[op][element]) and
+// shared by the parts, so the term loop reads LDS instead of re-reading L2
+// and the LDS of one tile serves several waves.  The terms of a
 // slot come in groups that share the set of modes on which they carry the
 // mode's identity operator: the product of those identity Grams is formed
 // once per group, and each term record multiplies only its remaining
@@ -658,34 +666,76 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
                                                           const int2* jobs,
                                                           int k) {
   extern __shared__ double Mc[];
-  int2 jb = jobs[blockIdx.x];
-  int E = blockDim.x;
-  int s = jb.x, R = a.R[s];
-  int e = jb.y * E + TID;
-  bool live = e < R * R;
-  int D = a.p.ndof;
-  const double* Ms = a.M + (long)s * a.nops * a.rstr + e;
-  if (live) {
-    for (int op = 0; op < a.nops; op++) Mc[op * E + TID] = Ms[(long)op * a.rstr];
-    Mc[a.nops * E + TID] = 1.0;
+  // workgroup = G groups of 64 elements x P parts (one wave each): wave w
+  // takes elements of group w % G and the slots [hsp[k][p], hsp[k][p+1]) of
+  // part p = w / G; all waves of a group share its staged Grams
+  const int2 jb = jobs[blockIdx.x];
+  const int P = a.nsplit, G = (blockDim.x >> 6) / P, E = 64 * G;
+  const int lane = TID & 63, w = __builtin_amdgcn_readfirstlane(TID >> 6);
+  const int grp = w % G, part = w / G;
+  const int s = jb.x, R = a.R[s], D = a.p.ndof;
+  const int e0 = jb.y * E, nel = min(E, R * R - e0);
+  const int el = grp * 64 + lane, e = e0 + el;
+  // stage the tile's operator Grams Mc[op][el], eight loads per thread in
+  // flight per round (clamped addresses, so every load is unconditional)
+  const double* Ms = a.M + (long)s * a.nops * a.rstr + e0;
+  const int tot = a.nops * E;
+  for (int i0 = 0; i0 < tot; i0 += 8 * (int)blockDim.x) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int idx = min(i0 + u * (int)blockDim.x + TID, tot - 1);
+      v[u] = Ms[(long)(idx / E) * a.rstr + min(idx % E, nel - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int idx = i0 + u * (int)blockDim.x + TID;
+      if (idx < tot) Mc[idx] = v[u];
+    }
   }
-  if (!live) return;
-  const double* Gs = a.G + (long)s * D * a.rstr + e;
-  double bb = 1.0;
-  for (int m = 1; m < D; m++) bb *= Gs[(long)((k + m) % D) * a.rstr];
-  a.BB[(long)s * a.rstr + e] = bb;
-  const double* mc = Mc + TID;
-  const int h0 = a.hs_off[k], h1 = a.hs_off[k + 1];
-  for (int u = h0; u < h1; u++) {
+  for (int i = TID; i < E; i += blockDim.x) Mc[a.nops * E + i] = 1.0;
+  __syncthreads();
+  if (el >= nel) return;
+  if (part == 0) {
+    // BB = prod_{m=1..D-1} G_{(k+m)%D}, all factors loaded together
+    const double* Gs = a.G + (long)s * D * a.rstr + e;
+    double gv[MLCP_MAXD];
+#pragma unroll
+    for (int m = 1; m < MLCP_MAXD; m++)
+      gv[m] = Gs[(long)((k + min(m, D - 1)) % D) * a.rstr];
+    double bb = 1.0;
+#pragma unroll
+    for (int m = 1; m < MLCP_MAXD; m++)
+      if (m < D) bb *= gv[m];
+    a.BB[(long)s * a.rstr + e] = bb;
+  }
+  const double* mc = Mc + el;
+  // The term tables are read-only for the launch and indexed uniformly: read
+  // them through the constant address space, so they come by scalar loads
+  // instead of a vector load (and its wait) per group of records.
+  const cptr<int> hs_off = (cptr<int>)(a.hs_off);
+  const cptr<int> hsp = (cptr<int>)(a.hsp);
+  const cptr<int> hs_beg = (cptr<int>)(a.hs_beg);
+  const cptr<int> hs_end = (cptr<int>)(a.hs_end);
+  const cptr<unsigned> hg_mask = (cptr<unsigned>)(a.hg_mask);
+  const cptr<int> hg_tb = (cptr<int>)(a.hg_tb);
+  const cptr<int> hg_te = (cptr<int>)(a.hg_te);
+  const cptr<int> hr_ops = (cptr<int>)(a.hr_ops);   // 4 per record
+  const cptr<double> hr_coef = (cptr<double>)(a.hr_coef);
+  const cptr<int> top = (cptr<int>)(a.top);
+  const int h0 = hs_off[k];
+  const int u0 = hsp[k * (a.nsplit + 1) + part];
+  const int u1 = hsp[k * (a.nsplit + 1) + part + 1];
+  for (int u = u0; u < u1; u++) {
     double acc = 0.0;
-    for (int g = a.hs_beg[u]; g < a.hs_end[u]; g++) {
-      const unsigned mask = a.hg_mask[g];
-      const int tb = a.hg_tb[g], te = a.hg_te[g];
+    for (int g = hs_beg[u]; g < hs_end[u]; g++) {
+      const unsigned mask = hg_mask[g];
+      const int tb = hg_tb[g], te = hg_te[g];
       if (mask >> 31) {
         // terms with more than four other operators: record x = term id
         for (int r = tb; r < te; r++) {
-          const int* to = a.top + (long)a.hr_ops[r].x * D;
-          double pr = a.hr_coef[r];
+          const cptr<int> to = top + (long)hr_ops[4 * r] * D;
+          double pr = hr_coef[r];
           for (int m = 1; m < D; m++) pr *= mc[to[(k + m) % D] * E];
           acc += pr;
         }
@@ -695,11 +745,11 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
       for (int d = 0; d < D; d++)
         if ((mask >> d) & 1u) ip *= mc[a.idop[d] * E];
       // absent factors read the entry holding 1, so the loop has no branch
-#pragma unroll 4
+#pragma unroll 8
       for (int r = tb; r < te; r++) {
-        const int4 o = a.hr_ops[r];
-        acc += a.hr_coef[r] * ip * ((mc[o.x * E] * mc[o.y * E]) *
-                                    (mc[o.z * E] * mc[o.w * E]));
+        const cptr<int> o = hr_ops + 4 * r;
+        acc += hr_coef[r] * ip * ((mc[o[0] * E] * mc[o[1] * E]) *
+                                  (mc[o[2] * E] * mc[o[3] * E]));
       }
     }
     a.W[((long)s * a.maxslots + (u - h0)) * a.rstr + e] = acc;

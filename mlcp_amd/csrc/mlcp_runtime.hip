@@ -16,6 +16,9 @@
 // Everything runs on one non-default stream of default priority.
 #include "mlcp_kernels.hip"
 #include <hip/hip_ext.h>
+#include <dlfcn.h>
+#include <fstream>
+#include <sstream>
 
 #include <algorithm>
 #include <cstdlib>
@@ -168,6 +171,11 @@ struct mlcp_session {
   // Gram-Schmidt
   DBuf<double> gsX, gsG, gsPart, gsBbs, gsPc, gsBjk, gsGjf;
   int had_tile = MLCP_NT;
+  // tunables (mlcp_hip_session_tune_set): Hadamard slot ranges per job and
+  // the Gram-Schmidt workgroup stride
+  int had_split = 1;
+  int gs_xst = 1;
+  DBuf<int> d_hsp;
   // linear-combination ALS
   DBuf<double> lcX, lcG, lcPart, lcW;
   DBuf<int2> lcJobs;
@@ -366,6 +374,64 @@ static int build_terms(mlcp_session* s, const int* opid, const int* mptr,
 // --------------------------------------------------------------------------
 // session setup
 
+// Workgroup stride of the Gram-Schmidt kernels: 8 keeps their work on one
+// XCD (workgroups are dealt to the eight XCDs round robin), 1 uses all CUs.
+// MLCP_GS_XCD_STRIDE sets a session's starting value (default 1); the
+// autotuner may change it per session.
+static int gs_stride() {
+  static int v = [] {
+    const char* e = getenv("MLCP_GS_XCD_STRIDE");
+    int x = e ? atoi(e) : 1;
+    return (x == 1 || x == 8) ? x : 1;
+  }();
+  return v;
+}
+
+// Largest Hadamard tile (multiple of 64, at most MLCP_NT) whose operator
+// Grams fit in LDS; 0 if not even 64 elements fit.
+static int had_tile_max(mlcp_session* s) {
+  const size_t per = (size_t)(s->nops + 1) * sizeof(double);
+  const size_t lim = lds_limit((const void*)k_op_hadamard);
+  for (int E = MLCP_NT; E >= 64; E -= 64)
+    if (per * E <= lim) return E;
+  return 0;
+}
+
+// Hadamard jobs (state, tile of had_tile elements; a workgroup runs
+// had_tile/64 x had_split waves) and, per mode, the slot ranges of the
+// parts: contiguous slots with about equal record counts.
+static int build_had_jobs(mlcp_session* s) {
+  const int D = s->ndof, P = s->had_split;
+  std::vector<int> hsp((size_t)D * (P + 1));
+  for (int k = 0; k < D; k++) {
+    const int h0 = s->hs_off[k], h1 = s->hs_off[k + 1];
+    long tot = 0;
+    for (int u = h0; u < h1; u++) {
+      for (int g = s->hs_beg[u]; g < s->hs_end[u]; g++)
+        tot += s->hg_te[g] - s->hg_tb[g];
+    }
+    int* b = hsp.data() + (size_t)k * (P + 1);
+    b[0] = h0;
+    long acc = 0;
+    int p = 1;
+    for (int u = h0; u < h1 && p < P; u++) {
+      for (int g = s->hs_beg[u]; g < s->hs_end[u]; g++)
+        acc += s->hg_te[g] - s->hg_tb[g];
+      while (p < P && acc * P >= tot * p) b[p++] = u + 1;
+    }
+    while (p < P) b[p++] = h1;
+    b[P] = h1;
+  }
+  std::vector<int2> had;
+  for (int i = 0; i < s->nbloc; i++) {
+    int nt = (s->ranks[i] * s->ranks[i] + s->had_tile - 1) / s->had_tile;
+    for (int t = 0; t < nt; t++) had.push_back(make_int2(i, t));
+  }
+  s->njobs_had = (int)had.size();
+  if (s->d_hsp.put(hsp, s->st) || s->jobs_had.put(had, s->st)) return -1;
+  return 0;
+}
+
 static int session_alloc(mlcp_session* s) {
   int D = s->ndof, B = s->nbloc;
   hipStream_t st = s->st;
@@ -420,23 +486,15 @@ static int session_alloc(mlcp_session* s) {
   if (s->jobs_all.put(all, st)) return -1;
   // Hadamard tile: elements per workgroup such that every operator's Gram
   // of the tile (and the entry holding 1) fits in LDS
-  {
-    size_t per = (size_t)(s->nops + 1) * sizeof(double);
-    const size_t lim = lds_limit((const void*)k_op_hadamard);
-    int E = MLCP_NT;
-    while (E > 64 && per * E > lim) E -= 64;
-    if (per * E > lim)
-      return fail("too many operators (" + std::to_string(s->nops) +
-                  ") for the LDS of one workgroup");
-    s->had_tile = E;
-  }
-  std::vector<int2> had;
-  for (int i = 0; i < B; i++) {
-    int nt = (s->ranks[i] * s->ranks[i] + s->had_tile - 1) / s->had_tile;
-    for (int t = 0; t < nt; t++) had.push_back(make_int2(i, t));
-  }
-  s->njobs_had = (int)had.size();
-  if (s->jobs_had.put(had, st)) return -1;
+  // default: one 64-element group shared by four waves (parts); the
+  // autotuner may choose otherwise
+  if (had_tile_max(s) < 64)
+    return fail("too many operators (" + std::to_string(s->nops) +
+                ") for the LDS of one workgroup");
+  s->had_tile = 64;
+  s->had_split = 4;
+  s->gs_xst = gs_stride();
+  if (build_had_jobs(s)) return -1;
   // Gram-Schmidt
   int pstr = s->Rmax * s->nmax;
   if (s->gsX.need((size_t)B * D * rstr) || s->gsG.need((size_t)D * rstr) ||
@@ -518,6 +576,8 @@ static OpArgs op_args(mlcp_session* s, bool work) {
   a.kord_off = s->d_kord_off.p;
   for (int d = 0; d < MLCP_MAXD; d++) a.idop[d] = s->idop[d];
   a.hs_off = s->d_hs_off.p;
+  a.hsp = s->d_hsp.p;
+  a.nsplit = s->had_split;
   a.hs_beg = s->d_hs_beg.p;
   a.hs_end = s->d_hs_end.p;
   a.hg_mask = s->d_hg_mask.p;
@@ -668,7 +728,7 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
   for (int it = 0; it < nitn; it++) {
     for (int k = 0; k < D; k++) {
       PLAUNCH(KID_OP_HADAMARD, k_op_hadamard, dim3(s->njobs_had),
-              dim3(s->had_tile),
+              dim3(s->had_tile * s->had_split),
               (size_t)(s->nops + 1) * s->had_tile * sizeof(double), a,
               (const int2*)s->jobs_had.p, k);
       size_t lds = ((size_t)s->Rmax * (s->nbas[k] | 1) +
@@ -704,23 +764,11 @@ static int phase_prodhv(mlcp_session* s, int nals) {
   return op_als(s, true, nals, 0, 0.0, false, false);
 }
 
-// Workgroup stride of the Gram-Schmidt kernels: 8 keeps their work on one
-// XCD (workgroups are dealt to the eight XCDs round robin), 1 uses all CUs.
-// MLCP_GS_XCD_STRIDE overrides the default.
-static int gs_stride() {
-  static int v = [] {
-    const char* e = getenv("MLCP_GS_XCD_STRIDE");
-    int x = e ? atoi(e) : 1;
-    return (x == 1 || x == 8) ? x : 1;
-  }();
-  return v;
-}
-
 static GsArgs gs_args(mlcp_session* s, int iv) {
   GsArgs g;
   g.p = s->pool;
   g.iv = iv;
-  g.xst = gs_stride();
+  g.xst = s->gs_xst;
   g.Riv = s->ranks[iv];
   g.cFiv = s->roff[iv];
   g.col = s->d_colQ.p;
@@ -948,6 +996,117 @@ static int validate_dims(const int* nbas, int ndof, int nbloc,
   return 0;
 }
 
+// --------------------------------------------------------------------------
+// Launch parameters (the autotuner's knobs) and the persisted tuning table
+
+// Problem key of the tuning table:
+// "<arch>/D<ndof>/n<n_1>.<n_2>.../B<nbloc>/R<min>.<max>.<sum>/o<nops>/t<T>".
+static std::string tune_key(mlcp_session* s) {
+  std::string arch = "unknown";
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+    arch = prop.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+  }
+  std::ostringstream k;
+  k << arch << "/D" << s->ndof << "/n";
+  for (int d = 0; d < s->ndof; d++) k << (d ? "." : "") << s->nbas[d];
+  const int rmin = *std::min_element(s->ranks.begin(), s->ranks.end());
+  k << "/B" << s->nbloc << "/R" << rmin << "." << s->Rmax << "." << s->Ntot
+    << "/o" << s->nops << "/t" << s->T;
+  return k.str();
+}
+
+static int tune_set(mlcp_session* s, int had_tile, int had_split, int gs_xst) {
+  const int tmax = had_tile_max(s);
+  if (had_tile && (had_tile % 64 || had_tile < 64 || had_tile > tmax))
+    return fail("had_tile must be a multiple of 64 in 64.." + std::to_string(tmax));
+  if (had_split && had_split != 1 && had_split != 2 && had_split != 4)
+    return fail("had_split must be 1, 2 or 4");
+  const int t = had_tile ? had_tile : s->had_tile;
+  const int p = had_split ? had_split : s->had_split;
+  if ((t / 64) * p * 64 > MLCP_NT)
+    return fail("had_tile / 64 * had_split must be at most " +
+                std::to_string(MLCP_NT / 64));
+  if (gs_xst && gs_xst != 1 && gs_xst != 8) return fail("gs_stride must be 1 or 8");
+  bool jobs = false;
+  if (had_tile && had_tile != s->had_tile) { s->had_tile = had_tile; jobs = true; }
+  if (had_split && had_split != s->had_split) { s->had_split = had_split; jobs = true; }
+  if (jobs && build_had_jobs(s)) return -1;
+  if (gs_xst && gs_xst != s->gs_xst) {
+    s->gs_xst = gs_xst;
+    if (s->gs_exec) {   // the captured Gram-Schmidt graph has the old grids
+      HCHK(hipGraphExecDestroy(s->gs_exec));
+      s->gs_exec = nullptr;
+    }
+  }
+  HCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+// Table: lines "<key> <had_tile> <had_split> <gs_stride>" ('#' comments).
+// The file is $MLCP_HIP_TUNING if set ("none": no table), else
+// tuning_table.txt beside this library.  A missing file or key leaves the
+// heuristic defaults.
+static int apply_tuning_table(mlcp_session* s) {
+  std::string path;
+  const char* e = getenv("MLCP_HIP_TUNING");
+  if (e) {
+    path = e;
+    if (path == "none") return 0;
+  } else {
+    Dl_info info;
+    if (!dladdr((const void*)&apply_tuning_table, &info) || !info.dli_fname)
+      return 0;
+    path = info.dli_fname;
+    const size_t sl = path.rfind('/');
+    path = (sl == std::string::npos ? std::string(".") : path.substr(0, sl)) +
+           "/tuning_table.txt";
+  }
+  std::ifstream f(path);
+  if (!f) return 0;
+  const std::string key = tune_key(s);
+  std::string line;
+  while (std::getline(f, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream ls(line);
+    std::string k;
+    int t = 0, p = 0, x = 0;
+    if (!(ls >> k >> t >> p >> x) || k != key) continue;
+    // an entry that does not fit this device (e.g. less LDS) is skipped
+    if (tune_set(s, t, p, x)) {
+      g_err.clear();
+      return 0;
+    }
+    return 0;
+  }
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_tune_set(mlcp_session* s, int had_tile,
+                                         int had_split, int gs_stride) {
+  g_err.clear();
+  return tune_set(s, had_tile, had_split, gs_stride);
+}
+
+extern "C" int mlcp_hip_session_tune_get(mlcp_session* s, int* out4) {
+  out4[0] = s->had_tile;
+  out4[1] = s->had_split;
+  out4[2] = s->gs_xst;
+  out4[3] = had_tile_max(s);
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_tune_key(mlcp_session* s, char* buf, int len) {
+  g_err.clear();
+  const std::string k = tune_key(s);
+  if (len < (int)k.size() + 1) return fail("key buffer too small");
+  memcpy(buf, k.c_str(), k.size() + 1);
+  return 0;
+}
+
 extern "C" const char* mlcp_hip_last_error(void) { return g_err.c_str(); }
 
 extern "C" int mlcp_hip_session_create(mlcp_session** out, const double* base,
@@ -980,6 +1139,12 @@ extern "C" int mlcp_hip_session_create(mlcp_session** out, const double* base,
   if (check_lds(s->Rmax, s->nmax, s->nbloc) ||
       build_terms(s, opid, mptr, hcoef, mat, T, nops, msize) ||
       session_alloc(s) || mlcp_hip_session_upload(s, base, coef)) {
+    std::string m = g_err;
+    delete s;
+    g_err = m;
+    return -1;
+  }
+  if (apply_tuning_table(s)) {
     std::string m = g_err;
     delete s;
     g_err = m;

@@ -227,6 +227,24 @@ class Session:
         u = np.ascontiguousarray(np.asarray(U, dtype=np.float64).T)
         check(lib().mlcp_hip_session_update(self.h, u, nals), "update")
 
+    def tune_set(self, had_tile=0, had_split=0, gs_stride=0):
+        """Launch parameters (0 keeps a value); results do not depend on
+        them bitwise."""
+        check(lib().mlcp_hip_session_tune_set(self.h, int(had_tile),
+                                              int(had_split), int(gs_stride)),
+              "tune_set")
+
+    def tuning(self) -> dict:
+        out = np.zeros(4, dtype=np.int32)
+        check(lib().mlcp_hip_session_tune_get(self.h, out), "tune_get")
+        return {"had_tile": int(out[0]), "had_split": int(out[1]),
+                "gs_stride": int(out[2]), "had_tile_max": int(out[3])}
+
+    def tune_key(self) -> str:
+        buf = C.create_string_buffer(512)
+        check(lib().mlcp_hip_session_tune_key(self.h, buf, 512), "tune_key")
+        return buf.value.decode()
+
 
 def spectral_probes(nbas, rk: int, seed: int):
     """The two start vectors of GetSpectralRange (Solver.f90:582-606): the

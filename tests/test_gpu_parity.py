@@ -38,6 +38,9 @@ CASES = [
     ("uniform", [5, 6, 4], [4, 4, 4], 5),
     ("ragged", [5, 6, 4], [3, 6, 4, 2], 6),
     ("fourmode", [4, 3, 5, 4], [5, 3, 4], 7),
+    # ranks above 32 (the four-row-tile factorisation and solve), with
+    # bases large enough that the normal equations stay definite
+    ("highrank", [16, 14, 18], [40, 20, 48, 33], 6),
 ]
 
 
@@ -269,3 +272,62 @@ def test_solve_restart_resumes(hip, tmp_path):
                      restart=True)
     assert rest.cycles == 3 and len(rest.history) == 1
     assert np.max(np.abs(rest.eigv - full.eigv)) < 1e-10 * np.abs(full.eigv).max()
+
+
+def test_tuning_is_bitwise_neutral(hip):
+    """The launch parameters (Hadamard tile and slot split, Gram-Schmidt
+    stride) change no result bit: a cycle from the same block gives the same
+    eigenvalues and factors under defaults and under other parameters."""
+    from mlcp_amd.intertwine import Session
+    say("tuning neutral")
+    nb = [5, 6, 4]
+    H = random_system(71, nb, ncoup=6)
+    Q = random_block(72, nb, [6, 3, 5, 4])
+    Q.eigv[:] = [1.0, 2.0, 3.0, 4.0]
+    with Session(Q, H) as s:
+        s.save()
+        d = s.tuning()
+        e0 = s.cycle(Q.eigv, 3, 3, True, 1, 5.0)
+        b0 = s.download()
+        for t, p, x in [(64, 1, 8), (128, 2, 1), (d["had_tile_max"], 1, 8)]:
+            s.restore()
+            s.tune_set(t, p, x)
+            assert s.tuning()["had_tile"] == t
+            e1 = s.cycle(Q.eigv, 3, 3, True, 1, 5.0)
+            b1 = s.download()
+            assert np.array_equal(e0, e1), (t, p, x)
+            assert np.array_equal(b0.base, b1.base) and np.array_equal(b0.coef, b1.coef)
+
+
+def test_tuning_rejects_bad_values(hip):
+    from mlcp_amd.intertwine import Session
+    say("tuning rejects")
+    nb = [5, 6, 4]
+    with Session(random_block(74, nb, [4, 4]), random_system(73, nb, ncoup=4)) as s:
+        for bad in [(100, 0, 0), (0, 3, 0), (0, 8, 0), (0, 0, 4), (1024, 0, 0),
+                    (256, 2, 0)]:
+            with pytest.raises(RuntimeError):
+                s.tune_set(*bad)
+
+
+def test_autotune_persists_and_applies(hip, tmp_path, monkeypatch):
+    """The tuner writes the chosen parameters under the session's key, and a
+    new session on the same problem starts from them (the library reads the
+    table named by MLCP_HIP_TUNING)."""
+    from mlcp_amd import autotune
+    from mlcp_amd.intertwine import Session
+    say("autotune")
+    nb = [5, 6, 4]
+    H = random_system(75, nb, ncoup=6)
+    Q = random_block(76, nb, [6, 3, 5, 4])
+    Q.eigv[:] = [1.0, 2.0, 3.0, 4.0]
+    table = str(tmp_path / "tune.txt")
+    monkeypatch.setenv("MLCP_HIP_TUNING", table)
+    with Session(Q, H) as s:
+        key = s.tune_key()
+        params, trials = autotune.tune(s, Q.eigv, 2, 2, 5.0, reps=1, table=table)
+    assert autotune.read_table(table)[key] == params
+    assert len(trials) > 2
+    with Session(Q, H) as s2:
+        t = s2.tuning()
+        assert (t["had_tile"], t["had_split"], t["gs_stride"]) == params
