@@ -58,16 +58,18 @@ def kernel_counts(w, kernel):
     n = H.nbas.astype(np.float64)
     N = float(R.sum())
     if kernel == "gs_solve":
-        # per Gram-Schmidt step of state iv on mode k (R = R_iv, n = n_k):
-        # Gauss-Jordan on [BB + pen | bjk] (R^3 + 2 R^2 n), own Gram of mode
-        # k (2 R^2 n), normalisation (3 R n) and renormalisation (4 R^2);
-        # bytes: bjk and BB read, factors and Gram written (8 B each)
+        # per Gram-Schmidt step of state iv on mode k (R = R_iv, n = n_k;
+        # the system matrix is factored beforehand by k_gs_reduce): the
+        # stored Gauss-Jordan steps applied to bjk (2 R^2 n), the Gram of the
+        # solution (R^2 n, one triangle), normalisation (2 R n) and
+        # renormalisation (4 R^2); bytes: bjk, the elimination steps (R^2)
+        # and BB read, factors and Gram written (8 B each)
         fl, by, cnt = 0.0, 0.0, 0
         for iv in range(1, Q.nbloc):
             r = R[iv]
             for nk in n:
-                fl += r ** 3 + 4 * r * r * nk + 3 * r * nk + 4 * r * r
-                by += 8 * (2 * r * nk + 2 * r * r)
+                fl += 3 * r * r * nk + 2 * r * nk + 4 * r * r
+                by += 8 * (2 * r * nk + 3 * r * r)
                 cnt += 1
         return fl / cnt, by / cnt
     return kernel_flops(w, kernel), None
