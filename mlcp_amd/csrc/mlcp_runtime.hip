@@ -99,7 +99,8 @@ static int setup_kernels() {
       (const void*)kern<32, m>
   const void* ks[] = {(const void*)k_op_refresh,   (const void*)k_op_hadamard,
                       NX_KERNELS(k_gs_terms),      NX_KERNELS(k_op_solve),
-                      NX_KERNELS2(k_gs_solve, 2),  NX_KERNELS2(k_gs_solve, 4),
+                      NX_KERNELS2(k_gs_solve, 2),  NX_KERNELS2(k_gs_solve, 3),
+                      NX_KERNELS2(k_gs_solve, 4),
                       NX_KERNELS(k_lc_solve)};
 #undef NX_KERNELS2
 #undef NX_KERNELS
@@ -848,6 +849,10 @@ static int launch_ortho(mlcp_session* s, int nals) {
           hipLaunchKernelGGL(k_gs_reduce<2>, dim3(nred * g.xst), dim3(MLCP_NT),
                              0, s->st, g, k);
           LAUNCH_NX2(KID_GS_SOLVE, k_gs_solve, 2, dim3(1), lds, g, k);
+        } else if (s->ranks[iv] <= 48) {
+          hipLaunchKernelGGL(k_gs_reduce<3>, dim3(nred * g.xst), dim3(MLCP_NT),
+                             0, s->st, g, k);
+          LAUNCH_NX2(KID_GS_SOLVE, k_gs_solve, 3, dim3(1), lds, g, k);
         } else {
           hipLaunchKernelGGL(k_gs_reduce<4>, dim3(nred * g.xst), dim3(MLCP_NT),
                              0, s->st, g, k);
