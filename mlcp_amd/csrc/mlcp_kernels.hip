@@ -757,84 +757,164 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
 }
 
 // Right-hand side, regularised solve and factor replacement for mode k of
-// every state.  One workgroup per state.  The right-hand side
+// every state (ALSPow.f90:604-629).  One workgroup per state.
 //   bjk(l,j) = sum_u sum_i W_u(i,l) c_i Y_u(j,i)  [- E sum_i BB(i,l) c_i F_i^k(j)]
-// is a sum of small GEMMs; each slot's W_u (scaled by c) and a 32-column
-// chunk of Y_u are staged in LDS, each thread accumulating 4 rows l.
-#define OPS_JCH 32
-template <int NX>
+// is a sum over the mode's slots of R x R by R x n products.  Wave 0
+// factors BB + pen I (gj_factor, elimination steps kept in LDS) while waves
+// 1..3 form the right-hand side on the FP64 matrix cores, column tiles
+// cb = g-1, g+2, ... of wave g, straight into the MFMA result layout (the
+// A operand c_i W_u(i,l) and the B operand Y_u(j,i) come from global
+// memory).  After one barrier every wave applies the steps to its own
+// tiles (no lane exchange); the row norms of the solution are summed across
+// lanes and waves, and the normalised rows replace the mode-k factors.
+template <int NX, int RTM>
 __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
                                                        int ishift,
                                                        double eshift) {
-  extern __shared__ double sm[];
-  __shared__ int flag;
-  __shared__ double gjs[GJ_SC(NX)];
-  int s = blockIdx.x;
-  int R = a.R[s], n = a.p.nbas[k], xs = n | 1;
-  double* X = sm;                    // R*xs, row l at X[l*xs]
-  double* Ws = sm + R * xs;          // R*R, Ws[i*R + l] = c_i W(i,l)
-  double* Ys = Ws + R * R;           // OPS_JCH*R, Ys[i*OPS_JCH + jj]
-  int u0 = a.kslot_off[k], u1 = a.kslot_off[k + 1];
-  const double* cold = a.p.coef + a.ocol[s];
+  constexpr int NCW = (NX / 4 + 2) / 3;   // column tiles per rhs wave
+  __shared__ double Es[GJF_SIZE];
+  __shared__ double fsc[GJF_SC];
+  __shared__ double nrm2[4 * 64];
+  const int s = blockIdx.x;
+  const int R = a.R[s], n = a.p.nbas[k];
+  const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
+  const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
+  const int RT = (R + 15) >> 4, CB = (n + 15) >> 4, nst = (R + 3) >> 2;
+  const v4d zero = {0.0, 0.0, 0.0, 0.0};
   const double* BB = a.BB + (long)s * a.rstr;
-  for (int e = TID; e < R * xs; e += MLCP_NT) X[e] = 0.0;
-  int nslot = u1 - u0 + (ishift != 0 ? 1 : 0);
-  int lq = (R + 3) / 4;              // groups of 4 rows
-  for (int q = 0; q < nslot; q++) {
-    bool shift = (q == u1 - u0);
-    const double* W =
-        shift ? BB : a.W + ((long)s * a.maxslots + q) * a.rstr;
-    double fac = shift ? -eshift : 1.0;
-    __syncthreads();
-    for (int e = TID; e < R * R; e += MLCP_NT) {
-      int i = e % R, l = e / R;
-      Ws[i * R + l] = fac * W[e] * cold[i];
-    }
-    const double* Y =
-        shift ? nullptr : a.Y + ((long)s * a.nops + a.kslot_op[u0 + q]) * a.ystr;
-    for (int j0 = 0; j0 < n; j0 += OPS_JCH) {
-      int nj = min(OPS_JCH, n - j0);
-      __syncthreads();
-      for (int e = TID; e < nj * R; e += MLCP_NT) {
-        int jj = e % nj, i = e / nj;
-        Ys[i * OPS_JCH + jj] =
-            shift ? colp(a.p, k, a.ocol[s] + i)[j0 + jj] : Y[j0 + jj + i * n];
-      }
-      __syncthreads();
-      for (int w = TID; w < lq * nj; w += MLCP_NT) {
-        int jj = w % nj, l0 = (w / nj) * 4;
-        double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
-        for (int i = 0; i < R; i++) {
-          double y = Ys[i * OPS_JCH + jj];
-          const double* wr = Ws + i * R + l0;
-          c0 += wr[0] * y;
-          if (l0 + 1 < R) c1 += wr[1] * y;
-          if (l0 + 2 < R) c2 += wr[2] * y;
-          if (l0 + 3 < R) c3 += wr[3] * y;
+  if (g == 0) {
+    v4d w[RTM][RTM];
+    gjf_load_bb<RTM>(w, BB, a.pen[s], R);
+    gj_factor<RTM>(w, R, Es, fsc);
+  }
+  v4d x[NCW][RTM];
+#pragma unroll
+  for (int j = 0; j < NCW; j++)
+#pragma unroll
+    for (int rt = 0; rt < RTM; rt++) x[j][rt] = zero;
+  if (g > 0) {
+    const double* cold = a.p.coef + a.ocol[s];
+    const int u0 = a.kslot_off[k], u1 = a.kslot_off[k + 1];
+    const int nslot = u1 - u0 + (ishift != 0 ? 1 : 0);
+    const int nks = (R + 3) >> 2;
+    for (int q = 0; q < nslot; q++) {
+      const bool shift = (q == u1 - u0);
+      const double* W = shift ? BB : a.W + ((long)s * a.maxslots + q) * a.rstr;
+      const double fac = shift ? -eshift : 1.0;
+      const double* Y = shift ? nullptr
+                              : a.Y + ((long)s * a.nops + a.kslot_op[u0 + q]) * a.ystr;
+      // k-steps of four i in batches of KB: all loads of a batch in flight
+      constexpr int KB = 8;
+      for (int ks0 = 0; ks0 < nks; ks0 += KB) {
+        double av[KB][RTM], bv[KB][NCW];
+#pragma unroll
+        for (int b = 0; b < KB; b++) {
+          const int i = 4 * (ks0 + b) + lq;
+          const int ic = min(i, R - 1);
+          const double ci = ld_or0(cold + ic, i < R) * fac;
+#pragma unroll
+          for (int rt = 0; rt < RTM; rt++) {
+            const int l = 16 * rt + lr;
+            av[b][rt] = ld_or0(W + ic + (long)min(l, R - 1) * R,
+                               rt < RT && l < R) * ci;
+          }
+#pragma unroll
+          for (int j = 0; j < NCW; j++) {
+            const int m = 16 * (g - 1 + 3 * j) + lr, mc = min(m, n - 1);
+            const double* yp = shift ? colp(a.p, k, a.ocol[s] + ic) + mc
+                                     : Y + mc + (long)ic * n;
+            bv[b][j] = ld_or0(yp, i < R && m < n);
+          }
         }
-        int j = j0 + jj;
-        X[l0 * xs + j] += c0;
-        if (l0 + 1 < R) X[(l0 + 1) * xs + j] += c1;
-        if (l0 + 2 < R) X[(l0 + 2) * xs + j] += c2;
-        if (l0 + 3 < R) X[(l0 + 3) * xs + j] += c3;
+#pragma unroll
+        for (int b = 0; b < KB; b++)
+          if (ks0 + b < nks) {
+#pragma unroll
+            for (int j = 0; j < NCW; j++)
+              if (g - 1 + 3 * j < CB) {
+#pragma unroll
+                for (int rt = 0; rt < RTM; rt++)
+                  if (rt < RT) x[j][rt] = mfma_f64_4(av[b][rt], bv[b][j], x[j][rt]);
+              }
+          }
       }
+    }
+    if (k == 0 && ishift < 0) {
+#pragma unroll
+      for (int j = 0; j < NCW; j++)
+#pragma unroll
+        for (int rt = 0; rt < RTM; rt++) x[j][rt] = -x[j][rt];
     }
   }
   __syncthreads();
-  if (k == 0 && ishift < 0)
-    for (int e = TID; e < R * xs; e += MLCP_NT) X[e] = -X[e];
-  double pen = a.pen[s];
-  double* A = Ws;
-  for (int e = TID; e < R * R; e += MLCP_NT) {
-    int i = e % R, l = e / R;
-    A[e] = BB[e] + (i == l ? pen : 0.0);
-  }
-  __syncthreads();
-  if (!wg_gj_solve<NX>(A, X, R, n, xs, gjs, &flag)) {
+  if (Es[16 * 256] != 0.0) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
-  wg_replace_mode(a.p, k, a.col[s], R, X, xs, a.err);
+  // X = elimination steps applied to the right-hand side tiles
+  if (g > 0) {
+#pragma unroll
+    for (int u = 0; u < 4 * RTM; u++) {
+      const int jb = u >> 2, u4 = u & 3;
+      double e[RTM];
+#pragma unroll
+      for (int rt = 0; rt < RTM; rt++) e[rt] = Es[(u * 4 + rt) * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < NCW; j++) {
+        if (u < nst && g - 1 + 3 * j < CB) {   // uniform
+          const double bro = x[j][jb][u4];
+#pragma unroll
+          for (int rt = 0; rt < RTM; rt++)
+            if (rt < RT) x[j][rt] = mfma_f64_4(e[rt], bro, x[j][rt]);
+        }
+      }
+    }
+  }
+  // row norms: element v of lane l is row 16rt + l/16 + 4v, column
+  // 16cb + l%16; sum over the 16 lanes of a row, then over the waves
+  double ps[RTM][4];
+#pragma unroll
+  for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < NCW; j++) t = fma(x[j][rt][v], x[j][rt][v], t);
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) t += __shfl_xor(t, o, 16);
+      ps[rt][v] = t;
+    }
+  if (lr == 0) {
+#pragma unroll
+    for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) nrm2[g * 64 + 16 * rt + lq + 4 * v] = ps[rt][v];
+  }
+  __syncthreads();
+  // coef_l = ||X(l,:)||, F_l^k = X(l,:) / coef_l (ALSPow.f90:624-629)
+  const int c0 = a.col[s];
+#pragma unroll
+  for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int l = 16 * rt + lq + 4 * v;
+      if (rt < RT && l < R) {
+        const double nr = sqrt(fabs((nrm2[64 + l] + nrm2[128 + l]) + nrm2[192 + l]));
+        if (g == 0 && lr == 0) {
+          a.p.coef[c0 + l] = nr;
+          if (nr != nr || nr == 0.0) atomicOr(a.err, 1);
+        }
+        if (g > 0) {
+          const double rn = 1.0 / nr;
+          double* f = colp(a.p, k, c0 + l);
+#pragma unroll
+          for (int j = 0; j < NCW; j++) {
+            const int m = 16 * (g - 1 + 3 * j) + lr;
+            if (m < n) f[m] = x[j][rt][v] * rn;
+          }
+        }
+      }
+    }
 }
 
 // c <- c / sqrt(|<F,F>|) with <F,F> from the cached per-mode Grams

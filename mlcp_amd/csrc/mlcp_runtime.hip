@@ -98,9 +98,10 @@ static int setup_kernels() {
       (const void*)kern<16, m>, (const void*)kern<24, m>,                   \
       (const void*)kern<32, m>
   const void* ks[] = {(const void*)k_op_refresh,   (const void*)k_op_hadamard,
-                      NX_KERNELS(k_gs_terms),      NX_KERNELS(k_op_solve),
+                      NX_KERNELS(k_gs_terms),
                       NX_KERNELS2(k_gs_solve, 2),  NX_KERNELS2(k_gs_solve, 3),
-                      NX_KERNELS2(k_gs_solve, 4),
+                      NX_KERNELS2(k_gs_solve, 4),  NX_KERNELS2(k_op_solve, 2),
+                      NX_KERNELS2(k_op_solve, 3),  NX_KERNELS2(k_op_solve, 4),
                       NX_KERNELS(k_lc_solve)};
 #undef NX_KERNELS2
 #undef NX_KERNELS
@@ -539,8 +540,6 @@ static int check_lds(int Rmax, int nmax, int nbloc) {
    : nx == 24 ? (const void*)kern<24, m> : (const void*)kern<32, m>)
   const bool bad =
       lds_ok(2 * n * r * b, "k_op_refresh", (const void*)k_op_refresh) ||
-      lds_ok((r * n1 + r * r + OPS_JCH * r) * b, "k_op_solve",
-             NX_PTR(k_op_solve)) ||
       lds_ok((r * n1 + r * r) * b, "k_lc_solve", NX_PTR(k_lc_solve)) ||
       lds_ok(65 * n * b, "k_gs_solve", NX_PTR2(k_gs_solve, 4)) ||
       lds_ok(256 * (size_t)nx * b, "k_gs_terms", NX_PTR(k_gs_terms));
@@ -732,11 +731,16 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
               dim3(s->had_tile * s->had_split),
               (size_t)(s->nops + 1) * s->had_tile * sizeof(double), a,
               (const int2*)s->jobs_had.p, k);
-      size_t lds = ((size_t)s->Rmax * (s->nbas[k] | 1) +
-                    (size_t)s->Rmax * s->Rmax + (size_t)OPS_JCH * s->Rmax) *
-                   sizeof(double);
-      LAUNCH_NX(KID_OP_SOLVE, k_op_solve, dim3(s->nbloc), lds, a, k, ishift,
-                eshift);
+      // row tiles of the largest rank of the block
+      if (s->Rmax <= 32)
+        LAUNCH_NX2(KID_OP_SOLVE, k_op_solve, 2, dim3(s->nbloc), 0, a, k,
+                   ishift, eshift);
+      else if (s->Rmax <= 48)
+        LAUNCH_NX2(KID_OP_SOLVE, k_op_solve, 3, dim3(s->nbloc), 0, a, k,
+                   ishift, eshift);
+      else
+        LAUNCH_NX2(KID_OP_SOLVE, k_op_solve, 4, dim3(s->nbloc), 0, a, k,
+                   ishift, eshift);
       bool last = (it == nitn - 1 && k == D - 1);
       if (!last || renorm) {
         PLAUNCH(KID_OP_REFRESH, k_op_refresh, dim3(s->njobs_mode[k]),
