@@ -590,61 +590,127 @@ template <typename T>
 using cptr = const __attribute__((address_space(4))) T*;
 
 // Refresh caches.  job = (s, op): op >= 0 -> M (and Y if do_y);
-// op = -(d+1) -> G_d.
+// op = -(d+1) -> G_d.  On the FP64 matrix cores: wave g owns column tile g
+// (CP terms i = 16g .. 16g+15 of state s) of Y = h F_old (n x R; identity
+// and diagonal operators scale F_old directly) for every 16-row tile of the
+// basis, in the MFMA result layout (element v of lane l: row 16mt + l/16 +
+// 4v, column 16g + l%16).  Element ks%4 of tile ks/4 is then exactly the B
+// operand of k-step ks of M = F^T Y (R x R, column tile g), so Y never
+// leaves the registers.  Waves past the state's rank return at once.
+template <int NX>
 __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
                                                          const int2* jobs,
                                                          int do_y) {
-  extern __shared__ double sm[];
-  int2 jb = jobs[blockIdx.x];
-  int s = jb.x, op = jb.y;
-  int d = op >= 0 ? a.opmode[op] : -op - 1;
-  int n = a.p.nbas[d], R = a.R[s];
-  double* Fs = sm;              // n*R, column i at Fs[i*n]
-  double* Ys = sm + n * R;      // n*R
-  for (int e = TID; e < n * R; e += MLCP_NT) {
-    int i = e / n, m = e % n;
-    Fs[e] = colp(a.p, d, a.col[s] + i)[m];
-  }
-  __syncthreads();
+  constexpr int MT = NX / 4;   // 16-row tiles of a basis of n <= 4 NX
+  const int2 jb = jobs[blockIdx.x];
+  const int s = jb.x, op = jb.y;
+  const int d = op >= 0 ? a.opmode[op] : -op - 1;
+  const int n = a.p.nbas[d], R = a.R[s];
+  const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
+  const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
+  const int RT = (R + 15) >> 4, NT = (n + 15) >> 4, nks = (n + 3) >> 2;
+  if (g >= RT) return;
+  const v4d zero = {0.0, 0.0, 0.0, 0.0};
+  const int i = 16 * g + lr;                 // this lane's column (term)
+  const bool iok = i < R;
+  const int ic = iok ? i : 0;
+  v4d y[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++) y[mt] = zero;
   if (op < 0) {
-    double* G = a.G + ((long)s * a.p.ndof + d) * a.rstr;
-    for (int e = TID; e < R * R; e += MLCP_NT) {
-      int i = e % R, l = e / R;
-      double acc = 0.0;
-      for (int m = 0; m < n; m++) acc += Fs[l * n + m] * Fs[i * n + m];
-      G[i + l * R] = acc;
-    }
-    return;
-  }
-  double* Yg = a.Y + ((long)s * a.nops + op) * a.ystr;
-  if (do_y) {
-    int cls = a.opcls[op];
-    const double* h = a.opd + a.opoff[op];
-    for (int e = TID; e < n * R; e += MLCP_NT) {
-      int i = e / n, m = e % n;
-      const double* fo = colp(a.p, d, a.ocol[s] + i);
-      double y;
-      if (cls == 0) {
-        y = fo[m];
-      } else if (cls == 1) {
-        y = h[m * n + m] * fo[m];
-      } else {
-        y = 0.0;
-        for (int j = 0; j < n; j++) y += h[m * n + j] * fo[j];
+    // G_d: Y = the current factor itself
+    const double* f = colp(a.p, d, a.col[s] + ic);
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int m = 16 * mt + lq + 4 * v;
+        y[mt][v] = ld_or0(f + min(m, n - 1), iok && m < n);
       }
-      Ys[e] = y;
-      Yg[e] = y;
-    }
+  } else if (!do_y) {
+    const double* Yg = a.Y + ((long)s * a.nops + op) * a.ystr;
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int m = 16 * mt + lq + 4 * v;
+        y[mt][v] = ld_or0(Yg + min(m, n - 1) + (long)ic * n, iok && m < n);
+      }
   } else {
-    for (int e = TID; e < n * R; e += MLCP_NT) Ys[e] = Yg[e];
+    const int cls = a.opcls[op];
+    const double* h = a.opd + a.opoff[op];
+    const double* fo = colp(a.p, d, a.ocol[s] + ic);
+    if (cls == 2) {
+      // dense: Y(m, i) = sum_j h[m*n + j] F_old_i[j], k-steps of four j,
+      // operands of eight k-steps in flight together
+      for (int ks0 = 0; ks0 < nks; ks0 += 8) {
+        double bv[8], av[8][MT];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+          const int j = 4 * (ks0 + b) + lq, jc = min(j, n - 1);
+          bv[b] = ld_or0(fo + jc, iok && j < n);
+#pragma unroll
+          for (int mt = 0; mt < MT; mt++) {
+            const int m = 16 * mt + lr;
+            av[b][mt] = ld_or0(h + (long)min(m, n - 1) * n + jc, m < n && j < n);
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+          if (ks0 + b < nks) {
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
+              if (mt < NT) y[mt] = mfma_f64_4(av[b][mt], bv[b], y[mt]);
+          }
+      }
+    } else {
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int m = 16 * mt + lq + 4 * v, mc = min(m, n - 1);
+          const double f = ld_or0(fo + mc, iok && m < n);
+          y[mt][v] = cls == 0 ? f : ld_or0(h + (long)mc * n + mc, m < n) * f;
+        }
+    }
+    double* Yg = a.Y + ((long)s * a.nops + op) * a.ystr;
+    if (iok) {
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int m = 16 * mt + lq + 4 * v;
+          if (m < n) Yg[m + (long)i * n] = y[mt][v];
+        }
+    }
   }
-  __syncthreads();
-  double* M = a.M + ((long)s * a.nops + op) * a.rstr;
-  for (int e = TID; e < R * R; e += MLCP_NT) {
-    int i = e % R, l = e / R;
-    double acc = 0.0;
-    for (int m = 0; m < n; m++) acc += Fs[l * n + m] * Ys[i * n + m];
-    M[i + l * R] = acc;
+  // M(l, i) = <F_l, Y_i> (or G(l, i) = <F_l, F_i>): row tile lt, this
+  // wave's column tile; A operand F_{16lt + l%16}[4ks + l/16]
+  double* out = op < 0 ? a.G + ((long)s * a.p.ndof + d) * a.rstr
+                       : a.M + ((long)s * a.nops + op) * a.rstr;
+  for (int lt = 0; lt < RT; lt++) {
+    const int la = 16 * lt + lr;
+    const double* fl = colp(a.p, d, a.col[s] + (la < R ? la : 0));
+    // k-step ks = 4 mt + v takes basis functions 4ks .. 4ks+3
+    double av[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int m = 16 * mt + 4 * v + lq;
+        av[mt][v] = ld_or0(fl + min(m, n - 1), la < R && m < n);
+      }
+    v4d acc = zero;
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        if (4 * mt + v < nks) acc = mfma_f64_4(av[mt][v], y[mt][v], acc);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int l = 16 * lt + lq + 4 * v;
+      if (l < R && iok) out[i + l * R] = acc[v];
+    }
   }
 }
 

@@ -97,7 +97,7 @@ static int setup_kernels() {
   (const void*)kern<8, m>, (const void*)kern<12, m>,                        \
       (const void*)kern<16, m>, (const void*)kern<24, m>,                   \
       (const void*)kern<32, m>
-  const void* ks[] = {(const void*)k_op_refresh,   (const void*)k_op_hadamard,
+  const void* ks[] = {(const void*)k_op_hadamard,  NX_KERNELS(k_op_refresh),
                       NX_KERNELS(k_gs_terms),
                       NX_KERNELS2(k_gs_solve, 2),  NX_KERNELS2(k_gs_solve, 3),
                       NX_KERNELS2(k_gs_solve, 4),  NX_KERNELS2(k_op_solve, 2),
@@ -539,7 +539,6 @@ static int check_lds(int Rmax, int nmax, int nbloc) {
    : nx == 16 ? (const void*)kern<16, m>                                    \
    : nx == 24 ? (const void*)kern<24, m> : (const void*)kern<32, m>)
   const bool bad =
-      lds_ok(2 * n * r * b, "k_op_refresh", (const void*)k_op_refresh) ||
       lds_ok((r * n1 + r * r) * b, "k_lc_solve", NX_PTR(k_lc_solve)) ||
       lds_ok(65 * n * b, "k_gs_solve", NX_PTR2(k_gs_solve, 4)) ||
       lds_ok(256 * (size_t)nx * b, "k_gs_terms", NX_PTR(k_gs_terms));
@@ -703,15 +702,9 @@ static int probe_collect(mlcp_session* s) {
 // --------------------------------------------------------------------------
 // phases
 
-static size_t lds_refresh(mlcp_session* s, int d) {
-  return (size_t)2 * s->nbas[d] * s->Rmax * sizeof(double);
-}
-
 static int refresh_all(mlcp_session* s, const OpArgs& a, int do_y) {
-  size_t lds = 0;
-  for (int d = 0; d < s->ndof; d++) lds = std::max(lds, lds_refresh(s, d));
-  PLAUNCH(KID_OP_REFRESH, k_op_refresh, dim3(s->njobs_all), dim3(MLCP_NT),
-          lds, a, (const int2*)s->jobs_all.p, do_y);
+  LAUNCH_NX(KID_OP_REFRESH, k_op_refresh, dim3(s->njobs_all), 0, a,
+            (const int2*)s->jobs_all.p, do_y);
   HCHK(hipGetLastError());
   return 0;
 }
@@ -743,9 +736,8 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
                    ishift, eshift);
       bool last = (it == nitn - 1 && k == D - 1);
       if (!last || renorm) {
-        PLAUNCH(KID_OP_REFRESH, k_op_refresh, dim3(s->njobs_mode[k]),
-                dim3(MLCP_NT), lds_refresh(s, k), a,
-                (const int2*)s->jobs_mode[k].p, do_y_each ? 1 : 0);
+        LAUNCH_NX(KID_OP_REFRESH, k_op_refresh, dim3(s->njobs_mode[k]), 0, a,
+                  (const int2*)s->jobs_mode[k].p, do_y_each ? 1 : 0);
       }
       if (renorm && k == D - 1)
         hipLaunchKernelGGL(k_op_renorm, dim3(s->nbloc), dim3(MLCP_NT), 0,
