@@ -57,19 +57,20 @@ def kernel_counts(w, kernel):
     R = Q.ranks.astype(np.float64)
     n = H.nbas.astype(np.float64)
     N = float(R.sum())
-    if kernel == "gs_solve":
-        # per Gram-Schmidt step of state iv on mode k (R = R_iv, n = n_k;
-        # the system matrix is factored beforehand by k_gs_reduce): the
-        # stored Gauss-Jordan steps applied to bjk (2 R^2 n), the Gram of the
-        # solution (R^2 n, one triangle), normalisation (2 R n) and
-        # renormalisation (4 R^2); bytes: bjk, the elimination steps (R^2)
-        # and BB read, factors and Gram written (8 B each)
+    if kernel == "gs_reduce":
+        # per Gram-Schmidt step of state iv on mode k (R = R_iv, n = n_k):
+        # the right-hand side from iv + 1 partials (2 iv R n), the
+        # factorisation of the R x R system matrix (R^3), its steps applied
+        # to the right-hand side (2 R^2 n), the Gram of the solution (R^2 n,
+        # one triangle), normalisation (2 R n) and renormalisation (4 R^2);
+        # bytes: partials read, BB and the elimination steps written and
+        # read, factors and Gram written (8 B each)
         fl, by, cnt = 0.0, 0.0, 0
         for iv in range(1, Q.nbloc):
             r = R[iv]
             for nk in n:
-                fl += 3 * r * r * nk + 2 * r * nk + 4 * r * r
-                by += 8 * (2 * r * nk + 3 * r * r)
+                fl += 2 * iv * r * nk + r ** 3 + 3 * r * r * nk + 2 * r * nk + 4 * r * r
+                by += 8 * ((iv + 1) * r * nk + 3 * r * nk + 4 * r * r)
                 cnt += 1
         return fl / cnt, by / cnt
     return kernel_flops(w, kernel), None
@@ -190,7 +191,7 @@ def main():
     sess.save()
     probe = args.probe
     if probe == "auto":
-        probe = "gs_solve"
+        probe = "gs_reduce"
 
     phases = np.zeros(5)
 

@@ -92,7 +92,8 @@ int mlcp_hip_session_save(mlcp_session *s);
 int mlcp_hip_session_restore(mlcp_session *s);
 /* Times every launch of one kernel with HIP events on the session's stream
  * (kernel ids: 1 op_hadamard, 2 op_solve, 3 op_refresh, 4 gs_terms,
- * 5 gs_solve, 6 lc_rhs, 7 lc_solve, 8 lc_refresh, 9 pair_inner, 10 gep;
+ * 5 gs_reduce (right-hand side, factorisation and solve of a Gram-Schmidt
+ * step), 6 lc_rhs, 7 lc_solve, 8 lc_refresh, 9 pair_inner, 10 gep;
  * 0 switches the probe off).  probe_read returns the summed milliseconds and
  * the number of timed launches since the last read. */
 int mlcp_hip_session_probe(mlcp_session *s, int kernel_id, int max_launches);

@@ -126,6 +126,7 @@ struct GsArgs {
   double* bbs;           // [rstr] prod_{d!=k} G_d
   double* bjk;           // [pstr] right-hand side of the step
   double* gjf;           // [GJF_SIZE] block Gauss-Jordan factors of the step
+  int* cnt;              // workgroups of k_gs_reduce done in this step (0 between steps)
   int* err;
 };
 // factor tiles (pivot block x row block x 256) and the pivot flag

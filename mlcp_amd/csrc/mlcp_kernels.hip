@@ -1232,85 +1232,24 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
   }
 }
 
-// Right-hand side of the Gram-Schmidt step, spread over workgroups (on the
-// same XCD as k_gs_terms): coef_l = -sum of the partial sums and
-// bjk = sum_{l<iv} coef_l P_l + P_iv (previous states first, ALSUtils.f90
-// order), for a slice of MLCP_NT elements per workgroup.  Workgroup 0
-// instead factors the step's system matrix for k_gs_solve (gj_factor on one
-// wave), which leaves the solve only the elimination of the right-hand side.
-template <int RTM>
-__global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k) {
-  __shared__ double cl[MLCP_BMAX];
-  __shared__ double fsc[GJF_SC];
-  if (blockIdx.x % a.xst) return;
-  int job = blockIdx.x / a.xst;
-  int iv = a.iv, R = a.Riv, n = a.p.nbas[k];
-  if (job == 0) {
-    // workgroup 0: factors of the step's system matrix BB + pen I (one wave)
-    if (TID < 64) {
-      v4d w[RTM][RTM];
-      gjf_load_bb<RTM>(w, a.bbs, *a.pen, R);
-      gj_factor<RTM>(w, R, a.gjf, fsc);
-    }
-    return;
-  }
-  job -= 1;
-  int nch = (R + GS_ICH - 1) / GS_ICH;
-  for (int l = TID; l < iv; l += MLCP_NT) {
-    double s = 0.0;
-    for (int c = 0; c < nch; c++) s += a.pc[(long)l * GS_PCS + c];
-    cl[l] = -s;
-  }
-  __syncthreads();
-  int e = job * MLCP_NT + TID;
-  if (e >= R * n) return;
-  // sixteen independent chains: sixteen partials in flight per round trip
-  constexpr int U = 16;
-  double sv[U];
-#pragma unroll
-  for (int u = 0; u < U; u++) sv[u] = 0.0;
-  const double* pe = a.part + e;
-  int l = 0;
-  for (; l + U <= iv; l += U) {
-    double pv[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) pv[u] = pe[(long)(l + u) * a.pstr];
-#pragma unroll
-    for (int u = 0; u < U; u++) sv[u] = fma(cl[l + u], pv[u], sv[u]);
-  }
-  {
-    // the last iv - l < U partials, from clamped rows with weight zero
-    double pv[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) pv[u] = pe[(long)min(l + u, iv) * a.pstr];
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      if (l + u < iv) sv[u] = fma(cl[l + u], pv[u], sv[u]);
-  }
-#pragma unroll
-  for (int w = U / 2; w > 0; w >>= 1)
-#pragma unroll
-    for (int u = 0; u < w; u++) sv[u] += sv[u + w];
-  a.bjk[e] = sv[0] + a.part[(long)iv * a.pstr + e];
-}
-
-// Solve of the Gram-Schmidt step: regularised solve with the right-hand side
-// of k_gs_reduce, factor replacement, refresh of the own Gram of mode k and
+// Solve of the Gram-Schmidt step (run by the last workgroup of
+// k_gs_reduce): regularised solve with its right-hand side, factor
+// replacement, refresh of the own Gram of mode k and
 // the renormalisation c <- c / sqrt(c^T (BB o G_k) c) (ALSUtils.f90:483-531).
-// The system matrix was factored by k_gs_reduce (gj_factor), so the solve
-// is the elimination applied to the right-hand side: wave g holds the 16-column tiles cb = g, g+4, ... of B for all row
+// The system matrix was factored by workgroup 0 of k_gs_reduce (gj_factor),
+// so the solve is the elimination applied to the right-hand side: wave g holds the 16-column tiles cb = g, g+4, ... of B for all row
 // tiles in registers and applies the ceil(R/4) steps with one MFMA per
 // tile each, needing no other wave.  The solution goes to T[m*65 + i] (65 n doubles of dynamic
 // LDS); the Gram X X^T over the basis (MFMA tiles, tj <= ti on wave t % 4)
 // gives the row norms coef_i = sqrt(G(i,i)), the new factors
 // F_i^k = X(i,:) / coef_i, G_k = G(i,j) / (coef_i coef_j) and the
 // renormalisation.
+// T: 65 n doubles of LDS, red: 8, cs and rcs: 64 each.
 template <int NX, int RTM>
-__global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
+__device__ __forceinline__ void gs_solve_body(const GsArgs& a, int k, double* T,
+                                              double* red, double* cs,
+                                              double* rcs) {
   constexpr int NCW = (NX / 4 + 3) / 4;   // column tiles of B per wave
-  extern __shared__ double T[];
-  __shared__ double red[8];
-  __shared__ double cs[64], rcs[64];
   const int R = a.Riv, cF = a.cFiv, n = a.p.nbas[k];
   const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
   const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
@@ -1476,6 +1415,91 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_solve(GsArgs a, int k) {
   const double tot = wg_sum(part, red);
   const double rs = 1.0 / sqrt(fabs(tot));
   if (TID < R) a.p.coef[cF + TID] = cs[TID] * rs;
+}
+
+// Right-hand side and solve of the Gram-Schmidt step.  Workgroups 1.. sum
+// the right-hand side, coef_l = -sum of the partial sums and
+// bjk = sum_{l<iv} coef_l P_l + P_iv (previous states first, ALSUtils.f90
+// order), a slice of MLCP_NT elements each; workgroup 0 factors the step's
+// system matrix BB + pen I (gj_factor on one wave).  Every workgroup then
+// makes its results visible device-wide and counts itself done; the last
+// one to finish (whichever it is) resets the counter and runs the solve
+// (gs_solve_body), so the step needs no separate solve launch and nothing
+// waits on another workgroup.  nj: participating workgroups; T: 65 n
+// doubles of dynamic LDS for the solve.
+template <int NX, int RTM>
+__global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k,
+                                                        int nj) {
+  extern __shared__ double T[];
+  __shared__ double cl[MLCP_BMAX];
+  __shared__ double fsc[GJF_SC];
+  __shared__ double red[8];
+  __shared__ double cs[64], rcs[64];
+  __shared__ int last;
+  if (blockIdx.x % a.xst) return;
+  const int job = blockIdx.x / a.xst;
+  const int iv = a.iv, R = a.Riv, n = a.p.nbas[k];
+  if (job == 0) {
+    // workgroup 0: factors of the step's system matrix BB + pen I (one wave)
+    if (TID < 64) {
+      v4d w[RTM][RTM];
+      gjf_load_bb<RTM>(w, a.bbs, *a.pen, R);
+      gj_factor<RTM>(w, R, a.gjf, fsc);
+    }
+  } else {
+    const int nch = (R + GS_ICH - 1) / GS_ICH;
+    for (int l = TID; l < iv; l += MLCP_NT) {
+      double s = 0.0;
+      for (int c = 0; c < nch; c++) s += a.pc[(long)l * GS_PCS + c];
+      cl[l] = -s;
+    }
+    __syncthreads();
+    const int e = (job - 1) * MLCP_NT + TID;
+    if (e < R * n) {
+      // sixteen independent chains: sixteen partials in flight per round
+      constexpr int U = 16;
+      double sv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) sv[u] = 0.0;
+      const double* pe = a.part + e;
+      int l = 0;
+      for (; l + U <= iv; l += U) {
+        double pv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) pv[u] = pe[(long)(l + u) * a.pstr];
+#pragma unroll
+        for (int u = 0; u < U; u++) sv[u] = fma(cl[l + u], pv[u], sv[u]);
+      }
+      {
+        // the last iv - l < U partials, from clamped rows with weight zero
+        double pv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) pv[u] = pe[(long)min(l + u, iv) * a.pstr];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (l + u < iv) sv[u] = fma(cl[l + u], pv[u], sv[u]);
+      }
+#pragma unroll
+      for (int w = U / 2; w > 0; w >>= 1)
+#pragma unroll
+        for (int u = 0; u < w; u++) sv[u] += sv[u + w];
+      a.bjk[e] = sv[0] + a.part[(long)iv * a.pstr + e];
+    }
+  }
+  // done: results visible device-wide (release), then count; the last
+  // workgroup resets the counter for the next step and, after an acquire
+  // fence, reads everyone's results
+  __syncthreads();
+  if (TID == 0) {
+    __threadfence();
+    const int done = atomicAdd(a.cnt, 1);
+    last = (done == nj - 1) ? 1 : 0;
+    if (last) atomicExch(a.cnt, 0);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  gs_solve_body<NX, RTM>(a, k, T, red, cs, rcs);
 }
 
 // --------------------------------------------------------------------------

@@ -99,8 +99,8 @@ static int setup_kernels() {
       (const void*)kern<32, m>
   const void* ks[] = {(const void*)k_op_hadamard,  NX_KERNELS(k_op_refresh),
                       NX_KERNELS(k_gs_terms),
-                      NX_KERNELS2(k_gs_solve, 2),  NX_KERNELS2(k_gs_solve, 3),
-                      NX_KERNELS2(k_gs_solve, 4),  NX_KERNELS2(k_op_solve, 2),
+                      NX_KERNELS2(k_gs_reduce, 2), NX_KERNELS2(k_gs_reduce, 3),
+                      NX_KERNELS2(k_gs_reduce, 4), NX_KERNELS2(k_op_solve, 2),
                       NX_KERNELS2(k_op_solve, 3),  NX_KERNELS2(k_op_solve, 4),
                       NX_KERNELS(k_lc_solve)};
 #undef NX_KERNELS2
@@ -172,6 +172,7 @@ struct mlcp_session {
   int njobs_all = 0, njobs_had = 0;
   // Gram-Schmidt
   DBuf<double> gsX, gsG, gsPart, gsBbs, gsPc, gsBjk, gsGjf;
+  DBuf<int> gsCnt;
   int had_tile = MLCP_NT;
   // tunables (mlcp_hip_session_tune_set): Hadamard slot ranges per job and
   // the Gram-Schmidt workgroup stride
@@ -502,7 +503,7 @@ static int session_alloc(mlcp_session* s) {
   if (s->gsX.need((size_t)B * D * rstr) || s->gsG.need((size_t)D * rstr) ||
       s->gsPart.need((size_t)(B + 1) * pstr) || s->gsBbs.need(rstr) ||
       s->gsPc.need((size_t)(B + 1) * GS_PCS) || s->gsBjk.need(pstr) ||
-      s->gsGjf.need(GJF_SIZE))
+      s->gsGjf.need(GJF_SIZE) || s->gsCnt.need(1))
     return -1;
   // GEP
   std::vector<PairJob> pj;
@@ -540,7 +541,7 @@ static int check_lds(int Rmax, int nmax, int nbloc) {
    : nx == 24 ? (const void*)kern<24, m> : (const void*)kern<32, m>)
   const bool bad =
       lds_ok((r * n1 + r * r) * b, "k_lc_solve", NX_PTR(k_lc_solve)) ||
-      lds_ok(65 * n * b, "k_gs_solve", NX_PTR2(k_gs_solve, 4)) ||
+      lds_ok(65 * n * b, "k_gs_reduce", NX_PTR2(k_gs_reduce, 4)) ||
       lds_ok(256 * (size_t)nx * b, "k_gs_terms", NX_PTR(k_gs_terms));
 #undef NX_PTR
 #undef NX_PTR2
@@ -657,7 +658,7 @@ static int check_err(mlcp_session* s, const char* phase) {
 
 enum {
   KID_OP_HADAMARD = 1, KID_OP_SOLVE, KID_OP_REFRESH, KID_GS_TERMS,
-  KID_GS_SOLVE, KID_LC_RHS, KID_LC_SOLVE, KID_LC_REFRESH, KID_PAIR_INNER,
+  KID_GS_REDUCE, KID_LC_RHS, KID_LC_SOLVE, KID_LC_REFRESH, KID_PAIR_INNER,
   KID_GEP
 };
 
@@ -781,6 +782,7 @@ static GsArgs gs_args(mlcp_session* s, int iv) {
   g.pc = s->gsPc.p;
   g.bjk = s->gsBjk.p;
   g.gjf = s->gsGjf.p;
+  g.cnt = s->gsCnt.p;
   g.err = s->err.p;
   return g;
 }
@@ -795,7 +797,7 @@ static bool use_graphs() {
 
 static int phase_ortho(mlcp_session* s, int nals) {
   if (nals < 1) return 0;
-  bool probing = s->probe_kid == KID_GS_TERMS || s->probe_kid == KID_GS_SOLVE;
+  bool probing = s->probe_kid == KID_GS_TERMS || s->probe_kid == KID_GS_REDUCE;
   if (!use_graphs() || probing) return launch_ortho(s, nals);
   if (!s->gs_exec || s->gs_nals != nals) {
     if (s->gs_exec) {
@@ -822,6 +824,9 @@ static int phase_ortho(mlcp_session* s, int nals) {
 
 static int launch_ortho(mlcp_session* s, int nals) {
   int D = s->ndof;
+  // the done-counter of k_gs_reduce starts every phase at zero (a fill
+  // kernel, so the phase can be captured)
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s->st, s->gsCnt.p, 1L, 0);
   hipLaunchKernelGGL(k_normcoef, dim3(1), dim3(MLCP_NT), 0, s->st, s->pool,
                      s->roff[0], s->ranks[0]);
   hipLaunchKernelGGL(k_penalty, dim3((s->nbloc + 63) / 64), dim3(64), 0,
@@ -838,22 +843,16 @@ static int launch_ortho(mlcp_session* s, int nals) {
         LAUNCH_NX(KID_GS_TERMS, k_gs_terms, dim3((iv + 1) * nch * g.xst),
                   lds_t, g, k, kprev);
         // workgroup 0 of k_gs_reduce factors the system matrix, the others
-        // sum the right-hand side; the solve applies the factors
+        // sum the right-hand side; the last one to finish solves
         const int nred = 1 + (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
         const size_t lds = (size_t)65 * s->nbas[k] * sizeof(double);
-        if (s->ranks[iv] <= 32) {
-          hipLaunchKernelGGL(k_gs_reduce<2>, dim3(nred * g.xst), dim3(MLCP_NT),
-                             0, s->st, g, k);
-          LAUNCH_NX2(KID_GS_SOLVE, k_gs_solve, 2, dim3(1), lds, g, k);
-        } else if (s->ranks[iv] <= 48) {
-          hipLaunchKernelGGL(k_gs_reduce<3>, dim3(nred * g.xst), dim3(MLCP_NT),
-                             0, s->st, g, k);
-          LAUNCH_NX2(KID_GS_SOLVE, k_gs_solve, 3, dim3(1), lds, g, k);
-        } else {
-          hipLaunchKernelGGL(k_gs_reduce<4>, dim3(nred * g.xst), dim3(MLCP_NT),
-                             0, s->st, g, k);
-          LAUNCH_NX2(KID_GS_SOLVE, k_gs_solve, 4, dim3(1), lds, g, k);
-        }
+        const dim3 gr(nred * g.xst);
+        if (s->ranks[iv] <= 32)
+          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 2, gr, lds, g, k, nred);
+        else if (s->ranks[iv] <= 48)
+          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 3, gr, lds, g, k, nred);
+        else
+          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 4, gr, lds, g, k, nred);
         kprev = k;
       }
   }

@@ -178,7 +178,7 @@ class Session:
         check(lib().mlcp_hip_session_restore(self.h), "session_restore")
 
     KERNELS = {"op_hadamard": 1, "op_solve": 2, "op_refresh": 3,
-               "gs_terms": 4, "gs_solve": 5, "lc_rhs": 6, "lc_solve": 7,
+               "gs_terms": 4, "gs_reduce": 5, "lc_rhs": 6, "lc_solve": 7,
                "lc_refresh": 8, "pair_inner": 9, "gep": 10}
 
     def probe(self, kernel: str, max_launches: int = 16384):
