@@ -1784,20 +1784,25 @@ __global__ __launch_bounds__(MLCP_NT) void k_pair_inner(Pool p,
 // parallel (round-robin ordered) two-sided Jacobi, ascending sort,
 // X = U^-1 V, sign: largest-|x| entry positive.  Work arrays in global memory
 // (ws: 4*n*n + 2*n doubles).  Output X column major n x n into Xo, w into wo.
+// MODE places the rotated matrix C and the accumulated rotations V: 0 both
+// in global memory, 1 C in LDS, 2 both in LDS (dynamic, n*n doubles each);
+// the host picks the largest that fits.
 #define GEP_U 4
+template <int MODE>
 __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
                                               const double* Sin, int n,
                                               double* ws, double* Xo,
                                               double* wo, int* err) {
+  extern __shared__ double gsh[];
   __shared__ int sflag;
   __shared__ double sred[32];
   __shared__ int pairs[2 * 128];
   __shared__ double rc[128], rs[128];
   int T = blockDim.x, tid = threadIdx.x;
   double* U = ws;
-  double* C = U + (long)n * n;
-  double* V = C + (long)n * n;
-  double* Z = V + (long)n * n;
+  double* C = MODE >= 1 ? gsh : U + (long)n * n;
+  double* V = MODE == 2 ? gsh + (long)n * n : U + 2 * (long)n * n;
+  double* Z = U + 3 * (long)n * n;
   double* wt = Z + (long)n * n;
   int* ord = (int*)(wt + n);
   // symmetric copies of the upper triangles

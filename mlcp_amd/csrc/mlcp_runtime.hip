@@ -102,7 +102,8 @@ static int setup_kernels() {
                       NX_KERNELS2(k_gs_reduce, 2), NX_KERNELS2(k_gs_reduce, 3),
                       NX_KERNELS2(k_gs_reduce, 4), NX_KERNELS2(k_op_solve, 2),
                       NX_KERNELS2(k_op_solve, 3),  NX_KERNELS2(k_op_solve, 4),
-                      NX_KERNELS(k_lc_solve)};
+                      NX_KERNELS(k_lc_solve),      (const void*)k_gep<1>,
+                      (const void*)k_gep<2>};
 #undef NX_KERNELS2
 #undef NX_KERNELS
   g_lds_dev = (size_t)v;
@@ -860,6 +861,28 @@ static int launch_ortho(mlcp_session* s, int nals) {
   return 0;
 }
 
+// LDS placement of the eigensolver's work matrices (k_gep MODE) for order n
+// and the dynamic LDS it needs.
+static int gep_mode(int n, size_t* lds) {
+  const size_t m = (size_t)n * n * sizeof(double);
+  if (2 * m <= lds_limit((const void*)k_gep<2>)) { *lds = 2 * m; return 2; }
+  if (m <= lds_limit((const void*)k_gep<1>)) { *lds = m; return 1; }
+  *lds = 0;
+  return 0;
+}
+
+#define LAUNCH_GEP(kid, n, ...)                                             \
+  do {                                                                    \
+    size_t lds_;                                                          \
+    const int mode_ = gep_mode(n, &lds_);                                 \
+    if (mode_ == 2)                                                       \
+      PLAUNCH(kid, k_gep<2>, dim3(1), dim3(1024), lds_, __VA_ARGS__);     \
+    else if (mode_ == 1)                                                  \
+      PLAUNCH(kid, k_gep<1>, dim3(1), dim3(1024), lds_, __VA_ARGS__);     \
+    else                                                                  \
+      PLAUNCH(kid, k_gep<0>, dim3(1), dim3(1024), lds_, __VA_ARGS__);     \
+  } while (0)
+
 static int phase_gep(mlcp_session* s) {
   int B = s->nbloc;
   hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, s->st, s->QHQ.p,
@@ -867,7 +890,7 @@ static int phase_gep(mlcp_session* s) {
   PLAUNCH(KID_PAIR_INNER, k_pair_inner, dim3(s->npj), dim3(MLCP_NT), 0,
           s->pool, (const PairJob*)s->pj.p, s->QHQ.p);
   // QHQ at QHQ.p + B*B, S at QHQ.p
-  PLAUNCH(KID_GEP, k_gep, dim3(1), dim3(1024), 0,
+  LAUNCH_GEP(KID_GEP, B,
           (const double*)(s->QHQ.p + (size_t)B * B), (const double*)s->QHQ.p,
           B, s->ws.p, s->U.p, s->eig.p, s->err.p);
   HCHK(hipGetLastError());
@@ -1441,8 +1464,18 @@ extern "C" int mlcp_hip_gep(int n, const double* A, const double* Bm,
   HCHK(hipMemcpy(dA, A, nn * sizeof(double), hipMemcpyHostToDevice));
   HCHK(hipMemcpy(dB, Bm, nn * sizeof(double), hipMemcpyHostToDevice));
   HCHK(hipMemset(e.p, 0, sizeof(int)));
-  hipLaunchKernelGGL(k_gep, dim3(1), dim3(1024), 0, 0, dA, dB, n, ws, dX, dw,
-                     e.p);
+  if (setup_kernels()) return -1;
+  size_t lds = 0;
+  const int mode = gep_mode(n, &lds);
+  if (mode == 2)
+    hipLaunchKernelGGL(k_gep<2>, dim3(1), dim3(1024), lds, 0, dA, dB, n, ws,
+                       dX, dw, e.p);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_gep<1>, dim3(1), dim3(1024), lds, 0, dA, dB, n, ws,
+                       dX, dw, e.p);
+  else
+    hipLaunchKernelGGL(k_gep<0>, dim3(1), dim3(1024), lds, 0, dA, dB, n, ws,
+                       dX, dw, e.p);
   HCHK(hipGetLastError());
   int err = 0;
   HCHK(hipMemcpy(&err, e.p, sizeof(int), hipMemcpyDeviceToHost));
