@@ -1518,71 +1518,107 @@ __global__ void k_lc_weights(double* w, const double* U, int ldU,
 
 // X[t][d](i,c) = <F_i^d, O_c^d> for a chunk of 64 old columns; chunk 0 also
 // refreshes G[t][d].  job = (t, chunk).  The cross Grams are an R x 64 x n
-// matrix product on the FP64 matrix cores: wave g owns rows [16g, 16g+16)
-// of the target and the four 16-column tiles of the chunk, the basis runs
-// in k-steps of 4 (operands loaded eight k-steps ahead).
+// matrix product on the FP64 matrix cores: wave g owns the 16 old columns
+// [16g, 16g+16) of the chunk for every row tile of the target, the basis
+// runs in k-steps of 4 (operands loaded eight k-steps ahead), and the result
+// tiles pass through the wave's LDS so that each column of X is written as
+// runs of consecutive rows.
+template <int RTM>
 __global__ __launch_bounds__(MLCP_NT) void k_lc_refresh(LcArgs a,
                                                          const int2* jobs,
                                                          int d) {
+  __shared__ double Ts[4][16 * 65];   // per wave: 16 columns x 64 rows (+1)
   int2 jb = jobs[blockIdx.x];
   int t = jb.x, ch = jb.y, R = a.R[t], n = a.p.nbas[d];
   int c0 = ch * LC_CHUNK, nc = min(LC_CHUNK, a.ncold - c0);
   double* X = a.X + ((long)t * a.p.ndof + d) * ((long)a.xr * a.ncold);
   const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
   const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
-  if (16 * g < R) {
-    const int ia = 16 * g + lr;
-    const double* f = colp(a.p, d, a.col[t] + (ia < R ? ia : 0));
-    const double* o[4];
-    bool cok[4];
+  const int RT = (R + 15) >> 4;
+  if (16 * g < nc) {
+    // wave g: old columns 16g .. 16g+15 of the chunk, every row tile
+    const int cl = 16 * g + lr;
+    const bool cok = cl < nc;
+    const double* o = colp(a.p, d, a.ocol0 + c0 + (cok ? cl : 0));
+    const double* f[RTM];
+    bool iok[RTM];
 #pragma unroll
-    for (int ct = 0; ct < 4; ct++) {
-      const int c = 16 * ct + lr;
-      cok[ct] = c < nc;
-      o[ct] = colp(a.p, d, a.ocol0 + c0 + (cok[ct] ? c : 0));
+    for (int rt = 0; rt < RTM; rt++) {
+      const int ia = 16 * rt + lr;
+      iok[rt] = rt < RT && ia < R;
+      f[rt] = colp(a.p, d, a.col[t] + (iok[rt] ? ia : 0));
     }
-    v4d acc[4];
+    v4d acc[RTM];
 #pragma unroll
-    for (int ct = 0; ct < 4; ct++) acc[ct] = (v4d){0.0, 0.0, 0.0, 0.0};
+    for (int rt = 0; rt < RTM; rt++) acc[rt] = (v4d){0.0, 0.0, 0.0, 0.0};
     const int nks = (n + 3) >> 2;
     for (int ks0 = 0; ks0 < nks; ks0 += 8) {
-      double av[8], bv[8][4];
+      double bv[8], av[8][RTM];
 #pragma unroll
       for (int b = 0; b < 8; b++) {
-        const int m = 4 * (ks0 + b) + lq;
-        const bool mok = m < n;
-        av[b] = (mok && ia < R) ? f[m] : 0.0;
+        const int m = 4 * (ks0 + b) + lq, mc = min(m, n - 1);
+        bv[b] = ld_or0(o + mc, cok && m < n);
 #pragma unroll
-        for (int ct = 0; ct < 4; ct++) bv[b][ct] = (mok && cok[ct]) ? o[ct][m] : 0.0;
+        for (int rt = 0; rt < RTM; rt++) av[b][rt] = ld_or0(f[rt] + mc, iok[rt] && m < n);
       }
 #pragma unroll
       for (int b = 0; b < 8; b++)
         if (ks0 + b < nks) {
 #pragma unroll
-          for (int ct = 0; ct < 4; ct++)
-            acc[ct] = mfma_f64_4(av[b], bv[b][ct], acc[ct]);
+          for (int rt = 0; rt < RTM; rt++)
+            if (rt < RT) acc[rt] = mfma_f64_4(av[b][rt], bv[b], acc[rt]);
         }
     }
-    // element v of lane l: row 16g + l/16 + 4v, column 16ct + l%16
+    // element v of lane l: row 16rt + l/16 + 4v, column 16g + l%16; the
+    // tile goes through this wave's LDS so that the stores run along rows
+    double* ts = Ts[g];
 #pragma unroll
-    for (int ct = 0; ct < 4; ct++) {
-      const int c = c0 + 16 * ct + lr;
+    for (int rt = 0; rt < RTM; rt++)
+      if (rt < RT) {
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int i = 16 * g + lq + 4 * v;
-        if (i < R && cok[ct]) X[(long)c * a.xr + i] = acc[ct][v];
+        for (int v = 0; v < 4; v++) ts[lr * 65 + 16 * rt + lq + 4 * v] = acc[rt][v];
       }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // the wave's 16 columns are one contiguous block of 16 xr doubles of X:
+    // store j writes its doubles 64j .. 64j + 63, one per lane
+    double* xb = X + (long)(c0 + 16 * g) * a.xr;
+    const int xr = a.xr, tot = min(16, nc - 16 * g) * xr;
+    for (int f = lane; f < tot; f += 64) {
+      const int col = f / xr, row = f - col * xr;
+      if (row < R) xb[f] = ts[col * 65 + row];
     }
   }
   if (ch == 0) {
+    // own Gram G(l, i) = <F_l^d, F_i^d>: tile (lt, it) of the RT x RT grid
+    // on wave (lt*RT + it) % 4
     double* G = a.G + ((long)t * a.p.ndof + d) * a.rstr;
-    for (int e = TID; e < R * R; e += MLCP_NT) {
-      int i = e % R, l = e / R;
-      const double* x = colp(a.p, d, a.col[t] + l);
-      const double* y = colp(a.p, d, a.col[t] + i);
-      double acc = 0.0;
-      for (int m = 0; m < n; m++) acc += x[m] * y[m];
-      G[i + l * R] = acc;
+    const int nks = (n + 3) >> 2;
+    for (int q = g; q < RT * RT; q += 4) {
+      const int lt = q / RT, it = q % RT;
+      const int la = 16 * lt + lr, ib = 16 * it + lr;
+      const double* fl = colp(a.p, d, a.col[t] + (la < R ? la : 0));
+      const double* fi = colp(a.p, d, a.col[t] + (ib < R ? ib : 0));
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      for (int ks0 = 0; ks0 < nks; ks0 += 8) {
+        double av[8], bv[8];
+#pragma unroll
+        for (int b2 = 0; b2 < 8; b2++) {
+          const int m = 4 * (ks0 + b2) + lq, mc = min(m, n - 1);
+          av[b2] = ld_or0(fl + mc, la < R && m < n);
+          bv[b2] = ld_or0(fi + mc, ib < R && m < n);
+        }
+#pragma unroll
+        for (int b2 = 0; b2 < 8; b2++)
+          if (ks0 + b2 < nks) acc = mfma_f64_4(av[b2], bv[b2], acc);
+      }
+      // element v: l = 16lt + l/16 + 4v, i = 16it + l%16
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int l = 16 * lt + lq + 4 * v;
+        if (l < R && ib < R) G[ib + l * R] = acc[v];
+      }
     }
   }
 }

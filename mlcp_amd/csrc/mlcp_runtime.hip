@@ -919,6 +919,17 @@ static LcArgs lc_args(mlcp_session* s, int B, const int* col, const int* R,
   return a;
 }
 
+// k_lc_refresh for the row tiles of the block's largest rank
+#define LC_REFRESH_LAUNCH(grid, block, lds, ...)                            \
+  do {                                                                    \
+    if (s->Rmax <= 32)                                                    \
+      PLAUNCH(KID_LC_REFRESH, k_lc_refresh<2>, grid, block, lds, __VA_ARGS__); \
+    else if (s->Rmax <= 48)                                               \
+      PLAUNCH(KID_LC_REFRESH, k_lc_refresh<3>, grid, block, lds, __VA_ARGS__); \
+    else                                                                  \
+      PLAUNCH(KID_LC_REFRESH, k_lc_refresh<4>, grid, block, lds, __VA_ARGS__); \
+  } while (0)
+
 // Linear-combination ALS sweeps on the prepared LcArgs (weights in lcW,
 // penalties in d_pen).
 static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
@@ -929,7 +940,7 @@ static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
   if (s->lcJobs.put(jobs, s->st)) return -1;
   int nj = (int)jobs.size();
   for (int d = 0; d < D; d++) {
-    PLAUNCH(KID_LC_REFRESH, k_lc_refresh, dim3(nj), dim3(MLCP_NT), 0, a,
+    LC_REFRESH_LAUNCH( dim3(nj), dim3(MLCP_NT), 0, a,
             (const int2*)s->lcJobs.p, d);
   }
   for (int it = 0; it < nitn; it++)
@@ -940,7 +951,7 @@ static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
                       (size_t)s->Rmax * s->Rmax) * sizeof(double);
       LAUNCH_NX(KID_LC_SOLVE, k_lc_solve, dim3(B), lds_s, a, k);
       if (!(it == nitn - 1 && k == D - 1)) {
-        PLAUNCH(KID_LC_REFRESH, k_lc_refresh, dim3(nj), dim3(MLCP_NT), 0, a,
+        LC_REFRESH_LAUNCH( dim3(nj), dim3(MLCP_NT), 0, a,
                 (const int2*)s->lcJobs.p, k);
       }
     }
