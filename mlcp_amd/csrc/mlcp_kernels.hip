@@ -725,9 +725,11 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
 // mode's identity operator: the product of those identity Grams is formed
 // once per group, and each term record multiplies only its remaining
 // (at most four) operators -- for a force field most factors of most terms
-// are identities.  Absent factors point at an LDS entry holding 1, so the
-// record loop is branch free.  Terms with more than four other operators
-// form groups of their own and take the plain product over all modes.
+// are identities.  All records of a group carry the same number of them,
+// so the group runs the loop for that count (reading only the operators it
+// has) and multiplies its sum by the identity product once.  Terms with more
+// than four other operators form groups of their own and take the plain
+// product over all modes.
 __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
                                                           const int2* jobs,
                                                           int k) {
@@ -810,13 +812,38 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
       double ip = 1.0;
       for (int d = 0; d < D; d++)
         if ((mask >> d) & 1u) ip *= mc[a.idop[d] * E];
-      // absent factors read the entry holding 1, so the loop has no branch
+      // every record of the group has the same number of explicit operators
+      // (the modes other than k without the identity); the group's sum is
+      // formed with exactly those and scaled by the identity product once
+      const int nx = (D - 1) - __popc(mask);
+      double ag = 0.0;
+      if (nx >= 4) {
 #pragma unroll 8
-      for (int r = tb; r < te; r++) {
-        const cptr<int> o = hr_ops + 4 * r;
-        acc += hr_coef[r] * ip * ((mc[o[0] * E] * mc[o[1] * E]) *
-                                  (mc[o[2] * E] * mc[o[3] * E]));
+        for (int r = tb; r < te; r++) {
+          const cptr<int> o = hr_ops + 4 * r;
+          ag = fma(hr_coef[r], (mc[o[0] * E] * mc[o[1] * E]) *
+                                   (mc[o[2] * E] * mc[o[3] * E]), ag);
+        }
+      } else if (nx == 3) {
+#pragma unroll 8
+        for (int r = tb; r < te; r++) {
+          const cptr<int> o = hr_ops + 4 * r;
+          ag = fma(hr_coef[r], (mc[o[0] * E] * mc[o[1] * E]) * mc[o[2] * E], ag);
+        }
+      } else if (nx == 2) {
+#pragma unroll 8
+        for (int r = tb; r < te; r++) {
+          const cptr<int> o = hr_ops + 4 * r;
+          ag = fma(hr_coef[r], mc[o[0] * E] * mc[o[1] * E], ag);
+        }
+      } else if (nx == 1) {
+#pragma unroll 8
+        for (int r = tb; r < te; r++)
+          ag = fma(hr_coef[r], mc[hr_ops[4 * r] * E], ag);
+      } else {
+        for (int r = tb; r < te; r++) ag += hr_coef[r];
       }
+      acc = fma(ip, ag, acc);
     }
     a.W[((long)s * a.maxslots + (u - h0)) * a.rstr + e] = acc;
   }
