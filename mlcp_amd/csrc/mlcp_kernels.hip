@@ -99,205 +99,6 @@ __device__ __forceinline__ void inv4_spd(const double (&P)[4][4],
     for (int c = r + 1; c < 4; c++) Q[r][c] = Q[c][r];
 }
 
-// Tile slots per wave of gj_mfma: the [A | B] system has at most
-// 4 x (4 + NX/4) tiles of 16 x 16 (R <= 64, n <= 4*NX).
-template <int NX>
-struct GjT {
-  static constexpr int S = 4 + NX / 4;   // tiles per wave
-  static constexpr int CB = 4 * 272;     // pivot column: 4 tiles, row stride 17
-  static constexpr int WS = 192;         // per-wave scratch of the 16 x 16 inverse
-};
-// LDS scratch (doubles) of gj_mfma: the pivot column tiles, the pivot row
-// tiles and the four waves' scratch.
-#define GJ_SC(NX) (GjT<NX>::CB + 256 * GjT<NX>::S + 4 * GjT<NX>::WS)
-
-// Geometry of the tiled system: row tiles RT = ceil(R/16), column tiles
-// CT = RT + ceil(n/16) (the matrix part first); tile t = rt*CT + ct lives in
-// slot t/4 of wave t%4.
-struct GjGeo {
-  int RT, CT, nt;
-  __device__ GjGeo(int R, int n)
-      : RT((R + 15) >> 4), CT(((R + 15) >> 4) + ((n + 15) >> 4)),
-        nt(((R + 15) >> 4) * (((R + 15) >> 4) + ((n + 15) >> 4))) {}
-};
-
-// Loads the tiles of wave g: matrix entries A(i, c) (identity beyond R, so
-// that padded pivot blocks stay invertible) and right-hand sides B(i, x)
-// (zero beyond R and n).  Tile element v of lane l is row 16rt + l/16 + 4v,
-// column 16ct + l%16 (the MFMA result layout).
-template <int NX, typename FA, typename FB>
-__device__ __forceinline__ void gjm_load(v4d (&w)[GjT<NX>::S], int R, int n,
-                                         FA A, FB B) {
-  const int lane = TID & 63, g = __builtin_amdgcn_readfirstlane(TID >> 6);
-  const int lr = lane & 15, lq = lane >> 4;
-  const GjGeo geo(R, n);
-#pragma unroll
-  for (int s = 0; s < GjT<NX>::S; s++) {
-    const int t = 4 * s + g;
-    const int rt = t / geo.CT, ct = t % geo.CT;
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      double val = 0.0;
-      if (t < geo.nt) {
-        const int i = 16 * rt + lq + 4 * v, c = 16 * ct + lr;
-        if (ct < geo.RT) {
-          val = (i < R && c < R) ? A(i, c) : (i == c ? 1.0 : 0.0);
-        } else {
-          const int x = c - 16 * geo.RT;
-          val = (i < R && x < n) ? B(i, x) : 0.0;
-        }
-      }
-      w[s][v] = val;
-    }
-  }
-}
-
-// Inverse of a 16 x 16 SPD tile held by one wave in the MFMA result layout
-// (element v of lane l is row l/16 + 4v, column l%16): Gauss-Jordan on
-// [P | I] in four steps of four pivots, each pivot block inverted by
-// inv4_spd, the pivot columns and rows exchanged through the wave's own LDS
-// scratch ws (192 doubles; one wave, so no workgroup barrier).  Returns
-// P^-1 in the same layout; bad is set if a pivot is not positive.
-__device__ __forceinline__ v4d inv16_spd(v4d w0, double* ws, bool& bad) {
-  const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
-  v4d w1;
-#pragma unroll
-  for (int v = 0; v < 4; v++) w1[v] = (lq + 4 * v == lr) ? 1.0 : 0.0;
-  double* colB = ws;        // [c*16 + row]
-  double* rowB = ws + 64;   // [r*32 + column of [P | I]]
-#pragma unroll
-  for (int u4 = 0; u4 < 4; u4++) {
-    const bool pcol = lr >= 4 * u4 && lr < 4 * u4 + 4;
-    if (pcol) {
-#pragma unroll
-      for (int v = 0; v < 4; v++) colB[(lr - 4 * u4) * 16 + lq + 4 * v] = w0[v];
-    }
-    rowB[lq * 32 + lr] = w0[u4];
-    rowB[lq * 32 + 16 + lr] = w1[u4];
-    __builtin_amdgcn_wave_barrier();
-    double P[4][4], Q[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-      for (int c = 0; c <= r; c++) P[r][c] = colB[c * 16 + 4 * u4 + r];
-    inv4_spd(P, Q, bad);
-    double prow[4];
-#pragma unroll
-    for (int c = 0; c < 4; c++)
-      prow[c] = lq == 0 ? Q[0][c] : lq == 1 ? Q[1][c] : lq == 2 ? Q[2][c] : Q[3][c];
-    double m = 0.0;
-#pragma unroll
-    for (int c = 0; c < 4; c++) m = fma(prow[c], colB[c * 16 + lr], m);
-    const double mneg = pcol ? 0.0 : -m;
-    const double b0 = rowB[lq * 32 + lr], b1 = rowB[lq * 32 + 16 + lr];
-    double f0 = 0.0, f1 = 0.0;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      f0 = fma(prow[c], rowB[c * 32 + lr], f0);
-      f1 = fma(prow[c], rowB[c * 32 + 16 + lr], f1);
-    }
-    __builtin_amdgcn_wave_barrier();
-    w0[u4] = f0;
-    w1[u4] = f1;
-    w0 = mfma_f64_4(mneg, b0, w0);
-    w1 = mfma_f64_4(mneg, b1, w1);
-  }
-  return w1;
-}
-
-// Solves A X = B (A SPD, R x R, R <= 64; n <= 4*NX right-hand sides) by
-// block Gauss-Jordan elimination without pivoting on 16 x 16 pivot blocks,
-// the system [A | B] held as 16 x 16 MFMA tiles in registers (gjm_load).
-// Per pivot block J (tile row/column jb): the owners publish the tiles of
-// column jb and of row jb to LDS (sc, GJ_SC(NX) doubles), barrier; every
-// wave inverts the pivot tile P itself (inv16_spd; all waves compute the
-// same bits), forms M(rt)^T = P^-1 A(rt,J)^T for the row tiles it updates
-// (P^-1 is symmetric, so its result-layout elements are its own A operands
-// and M^T in the result layout is the A operand of M), and updates its live
-// tiles (column > jb): rows J become P^-1 W(J,:), the others
-// W <- W - M W(J,:), four MFMA k-steps each; barrier.  On exit the
-// right-hand-side tiles hold X.  Returns false on every thread if a pivot
-// is not positive.
-template <int NX>
-__device__ __forceinline__ bool gj_mfma(v4d (&w)[GjT<NX>::S], int R, int n,
-                                        double* sc, int* flag) {
-  constexpr int S = GjT<NX>::S, CB = GjT<NX>::CB;
-  const int lane = TID & 63, g = __builtin_amdgcn_readfirstlane(TID >> 6);
-  const int lr = lane & 15, lq = lane >> 4;
-  const GjGeo geo(R, n);
-  double* colT = sc;                          // tile rt: [rt*272 + row*17 + col]
-  double* rowT = sc + CB;                     // tile ct: [ct*256 + v*64 + lane]
-  double* ws = sc + CB + 256 * S + g * GjT<NX>::WS;
-  int srt[S], sct[S];
-#pragma unroll
-  for (int s = 0; s < S; s++) {
-    const int t = 4 * s + g;
-    srt[s] = t < geo.nt ? t / geo.CT : -1;
-    sct[s] = t < geo.nt ? t % geo.CT : -1;
-  }
-  bool bad = false;
-  const v4d zero = {0.0, 0.0, 0.0, 0.0};
-  for (int jb = 0; jb < geo.RT; jb++) {
-#pragma unroll
-    for (int s = 0; s < S; s++) {
-      if (sct[s] == jb) {
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-          colT[srt[s] * 272 + (lq + 4 * v) * 17 + lr] = w[s][v];
-      }
-      if (srt[s] == jb && sct[s] > jb) {
-#pragma unroll
-        for (int v = 0; v < 4; v++) rowT[sct[s] * 256 + v * 64 + lane] = w[s][v];
-      }
-    }
-    __syncthreads();
-    v4d pv;
-#pragma unroll
-    for (int v = 0; v < 4; v++) pv[v] = colT[jb * 272 + (lq + 4 * v) * 17 + lr];
-    const v4d pinv = inv16_spd(pv, ws, bad);
-    // M(rt)^T for the row tiles of this wave's live tiles
-    v4d mt[4];
-#pragma unroll
-    for (int rt = 0; rt < 4; rt++) {
-      bool need = false;
-#pragma unroll
-      for (int s = 0; s < S; s++) need |= (srt[s] == rt && sct[s] > jb);
-      mt[rt] = zero;
-      if (need && rt != jb) {
-        const double* ct_ = colT + rt * 272 + lr * 17 + lq;
-#pragma unroll
-        for (int ks = 0; ks < 4; ks++) mt[rt] = mfma_f64_4(pinv[ks], ct_[4 * ks], mt[rt]);
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < S; s++) {
-      const int rt = srt[s], ct = sct[s];
-      if (ct > jb) {
-        const double* rb = rowT + ct * 256 + lane;
-        double bk[4];
-#pragma unroll
-        for (int ks = 0; ks < 4; ks++) bk[ks] = rb[ks * 64];
-        if (rt == jb) {
-          v4d acc = zero;
-#pragma unroll
-          for (int ks = 0; ks < 4; ks++) acc = mfma_f64_4(pinv[ks], bk[ks], acc);
-          w[s] = acc;
-        } else {
-          const v4d m = rt == 0 ? mt[0] : rt == 1 ? mt[1] : rt == 2 ? mt[2] : mt[3];
-#pragma unroll
-          for (int ks = 0; ks < 4; ks++) w[s] = mfma_f64_4(-m[ks], bk[ks], w[s]);
-        }
-      }
-    }
-    __syncthreads();
-  }
-  if (TID == 0) *flag = bad ? 1 : 0;
-  __syncthreads();
-  bool ok = (*flag == 0);
-  __syncthreads();
-  return ok;
-}
-
 __device__ __forceinline__ int clampi(int x, int hi) {
   return x < 0 ? 0 : (x > hi ? hi : x);
 }
@@ -422,34 +223,74 @@ __device__ __forceinline__ void gj_factor(v4d (&w)[RTM][RTM], int R, double* f,
   if (lane == 0) f[16 * 256] = bad ? 1.0 : 0.0;
 }
 
-// Solves A X = B for an SPD R x R matrix A (column major, LDS) and n
-// right-hand sides stored as X[l*xs + j] (LDS, overwritten by the solution)
-// through gj_mfma.  sc holds GJ_SC(NX) doubles.
-template <int NX>
-__device__ __forceinline__ bool wg_gj_solve(const double* A, double* X, int R,
-                                            int n, int xs, double* sc,
-                                            int* flag) {
-  v4d w[GjT<NX>::S];
-  gjm_load<NX>(w, R, n, [&](int r, int c) { return A[r + c * R]; },
-               [&](int r, int c) { return X[r * xs + c]; });
+// Solves A X = B for the symmetric positive definite R x R matrix A (row
+// stride R, R <= 64, LDS) and n right-hand sides X[l*xs + j] (LDS,
+// overwritten by the solution) by A = L D L^T and substitution, the
+// backward-stable counterpart of the reference's LU solve (getrf/getrs,
+// linalg.cu:73-111): the computed factors are exact for A + dA with
+// |dA| ~ R eps |A|, so the pivots stay positive whenever the penalised
+// normal equations are, however ill-conditioned (pivot blocks inverted
+// explicitly lose up to eps cond(A) |A| in every Schur complement).
+//   factor  one pivot per step, right-looking on the whole trailing block:
+//           wave g updates its rows j+1+g, j+5+g, ... (lane = column) with
+//           the unscaled pivot column, so every step has one barrier; a last
+//           pass scales the strictly lower part into L;
+//   solve   thread per right-hand side: L y = b (forward), D z = y,
+//           L^T x = z (backward), four partial sums per row.
+// Returns false on every thread if a pivot is not positive.
+__device__ bool wg_ldl_solve(double* A, double* X, int R, int n, int xs,
+                             int* flag) {
+  const int lane = TID & 63, g = TID >> 6;
+  if (TID == 0) *flag = 0;
+  for (int j = 0; j + 1 < R; j++) {
+    const double rd = 1.0 / A[j * R + j];
+    const int k = j + 1 + lane;
+    const double ajk = k < R ? A[j * R + k] : 0.0;
+    for (int i = j + 1 + g; i < R; i += MLCP_NT >> 6) {
+      const double lij = A[i * R + j] * rd;
+      if (k < R) A[i * R + k] = fma(-lij, ajk, A[i * R + k]);
+    }
+    __syncthreads();
+  }
+  for (int j = TID; j < R; j += MLCP_NT)
+    if (!(A[j * R + j] > 0.0)) *flag = 1;
+  for (int e = TID; e < R * R; e += MLCP_NT) {
+    const int i = e / R, j = e - i * R;
+    if (i > j) A[e] = A[e] / A[j * R + j];
+  }
   __syncthreads();
-  const bool ok = gj_mfma<NX>(w, R, n, sc, flag);
-  const int lane = TID & 63, g = TID >> 6, lr = lane & 15, lq = lane >> 4;
-  const GjGeo geo(R, n);
-#pragma unroll
-  for (int s = 0; s < GjT<NX>::S; s++) {
-    const int t = 4 * s + g;
-    const int rt = t / geo.CT, ct = t % geo.CT;
-    if (t < geo.nt && ct >= geo.RT) {
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int i = 16 * rt + lq + 4 * v, x = 16 * (ct - geo.RT) + lr;
-        if (i < R && x < n) X[i * xs + x] = w[s][v];
+  if (*flag) return false;
+  for (int c = TID; c < n; c += MLCP_NT) {
+    double* x = X + c;
+    for (int i = 1; i < R; i++) {
+      const double* li = A + i * R;
+      double s0 = x[i * xs], s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int j = 0;
+      for (; j + 4 <= i; j += 4) {
+        s0 = fma(-li[j], x[j * xs], s0);
+        s1 = fma(-li[j + 1], x[(j + 1) * xs], s1);
+        s2 = fma(-li[j + 2], x[(j + 2) * xs], s2);
+        s3 = fma(-li[j + 3], x[(j + 3) * xs], s3);
       }
+      for (; j < i; j++) s0 = fma(-li[j], x[j * xs], s0);
+      x[i * xs] = (s0 + s1) + (s2 + s3);
+    }
+    for (int i = 0; i < R; i++) x[i * xs] = x[i * xs] / A[i * R + i];
+    for (int i = R - 2; i >= 0; i--) {
+      double s0 = x[i * xs], s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int j = i + 1;
+      for (; j + 4 <= R; j += 4) {
+        s0 = fma(-A[j * R + i], x[j * xs], s0);
+        s1 = fma(-A[(j + 1) * R + i], x[(j + 1) * xs], s1);
+        s2 = fma(-A[(j + 2) * R + i], x[(j + 2) * xs], s2);
+        s3 = fma(-A[(j + 3) * R + i], x[(j + 3) * xs], s3);
+      }
+      for (; j < R; j++) s0 = fma(-A[j * R + i], x[j * xs], s0);
+      x[i * xs] = (s0 + s1) + (s2 + s3);
     }
   }
   __syncthreads();
-  return ok;
+  return true;
 }
 
 // "Construct improved F" (e.g. ALSPow.f90:624-629): coef_l = ||X(l,:)||,
@@ -1724,11 +1565,12 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
   }
 }
 
-template <int NX>
+// mode-k step of target t (one workgroup): right-hand side = sum of the chunk
+// partials, matrix = prod_{d != k} G_d + pen I, solved by wg_ldl_solve; then
+// the normalised rows replace mode k and its Gram is refreshed
 __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
   extern __shared__ double sm[];
   __shared__ int flag;
-  __shared__ double gjs[GJ_SC(NX)];
   int t = blockIdx.x, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof, xs = n | 1;
   double* X = sm;
   double* A = sm + R * xs;
@@ -1755,7 +1597,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
     A[e] = bb + (i == l ? pen : 0.0);
   }
   __syncthreads();
-  if (!wg_gj_solve<NX>(A, X, R, n, xs, gjs, &flag)) {
+  if (!wg_ldl_solve(A, X, R, n, xs, &flag)) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }

@@ -127,6 +127,35 @@ def test_update_phase(hip, name, nb, ranks, ncoup):
         assert e < TOL_T, (i, e)
 
 
+def test_update_phase_ill_conditioned(hip):
+    """The update of the 3-mode Henon-Heiles top block after one power +
+    Gram-Schmidt pass: during its ALS sweeps the normal equations reach
+    condition numbers near 2e10 (smallest eigenvalue ~7e-10 against a penalty
+    of 1e-10).  The solve must keep them positive definite and agree with the
+    oracle's LU solve; the tolerance is the represented-tensor bound widened
+    by that conditioning."""
+    from mlcp_amd import systems
+    from mlcp_amd.intertwine import Session, block_shift
+    from oracle.orc import C
+    w = systems.hh111()
+    H, Q = w.H, w.Q0.copy()
+    say("update ill-conditioned", list(Q.nbas), int(Q.ranks.max()), Q.nbloc)
+    es = block_shift(w.Q0.eigv, w.bounds)
+    for i in range(Q.nbloc):
+        c, b = C.als_pow3(H, *Q.state(i), w.npow, 1, es)
+        qc, qb = Q.state(i)
+        qc[:], qb[:] = c, b
+    Q = C.als_ortho3(Q, w.nals)
+    _, U = C.dsygv(C.getqhq3(H, Q, w.nals), C.getoverlaps(Q))
+    with Session(Q, H) as s:
+        s.update(U, w.nals)
+        g = s.download()
+    for i in range(Q.nbloc):
+        c, b = C.sumlcvec3(*Q.state(i), Q, U[:, i], w.nals)
+        e = rel(full_tensor(Q.nbas, *g.state(i)), full_tensor(Q.nbas, c, b))
+        assert e < 1e-7, (i, e)
+
+
 @pytest.mark.parametrize("n", [1, 2, 7, 33, 64, 130, 200])
 def test_gep_solver(hip, n):
     from mlcp_amd.intertwine import gpu_gep
