@@ -130,4 +130,8 @@ struct GsArgs {
   int* err;
 };
 // factor tiles (pivot block x row block x 256) and the pivot flag
-#define GJF_SIZE (16 * 256 + 64)
+// block Gauss-Jordan factors: 16 steps x 4 row tiles of E_u, 16 tiles of
+// E1_u, the flag
+#define GJF_E1 (16 * 256)
+#define GJF_FLAG (16 * 320)
+#define GJF_SIZE (16 * 320 + 64)

@@ -60,11 +60,12 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return fma(r, e, r);
 }
 
-// Inverse of a 4 x 4 SPD block from its lower triangle (P[r][c], r >= c)
-// through P = L D L^T:  P^-1 = L^-T D^-1 L^-1.  bad is set if a pivot of D
-// is not positive.
-__device__ __forceinline__ void inv4_spd(const double (&P)[4][4],
-                                         double (&Q)[4][4], bool& bad) {
+// P = L D L^T of a 4 x 4 SPD block from its lower triangle (P[r][c],
+// r >= c): the unit lower inverse Li = L^-1 and dinv = D^-1.  bad is set if
+// a pivot of D is not positive.
+__device__ __forceinline__ void ldl4_spd(const double (&P)[4][4],
+                                         double (&Li)[4][4], double (&dinv)[4],
+                                         bool& bad) {
   const double d0 = P[0][0], r0 = rcp_nr(d0);
   const double l10 = P[1][0] * r0, l20 = P[2][0] * r0, l30 = P[3][0] * r0;
   const double s11 = P[1][1] - l10 * P[1][0], s21 = P[2][1] - l20 * P[1][0];
@@ -78,25 +79,15 @@ __device__ __forceinline__ void inv4_spd(const double (&P)[4][4],
   const double l32 = t32 * r2;
   const double d3 = t33 - l32 * t32, r3 = rcp_nr(d3);
   bad |= !(d0 > 0.0 && d1 > 0.0 && d2 > 0.0 && d3 > 0.0);
-  // unit lower L^-1
   const double n10 = -l10, n21 = -l21, n32 = -l32;
   const double n20 = -l20 - l21 * n10;
   const double n31 = -l31 - l32 * n21;
   const double n30 = -l30 - l31 * n10 - l32 * n20;
-  Q[3][3] = r3;
-  Q[3][2] = r3 * n32;
-  Q[3][1] = r3 * n31;
-  Q[3][0] = r3 * n30;
-  Q[2][2] = r2 + r3 * n32 * n32;
-  Q[2][1] = r2 * n21 + r3 * n32 * n31;
-  Q[2][0] = r2 * n20 + r3 * n32 * n30;
-  Q[1][1] = r1 + r2 * n21 * n21 + r3 * n31 * n31;
-  Q[1][0] = r1 * n10 + r2 * n21 * n20 + r3 * n31 * n30;
-  Q[0][0] = r0 + r1 * n10 * n10 + r2 * n20 * n20 + r3 * n30 * n30;
-#pragma unroll
-  for (int r = 0; r < 4; r++)
-#pragma unroll
-    for (int c = r + 1; c < 4; c++) Q[r][c] = Q[c][r];
+  Li[0][0] = 1.0; Li[0][1] = 0.0; Li[0][2] = 0.0; Li[0][3] = 0.0;
+  Li[1][0] = n10; Li[1][1] = 1.0; Li[1][2] = 0.0; Li[1][3] = 0.0;
+  Li[2][0] = n20; Li[2][1] = n21; Li[2][2] = 1.0; Li[2][3] = 0.0;
+  Li[3][0] = n30; Li[3][1] = n31; Li[3][2] = n32; Li[3][3] = 1.0;
+  dinv[0] = r0; dinv[1] = r1; dinv[2] = r2; dinv[3] = r3;
 }
 
 __device__ __forceinline__ int clampi(int x, int hi) {
@@ -147,18 +138,26 @@ __device__ __forceinline__ void gjf_load_bb(v4d (&w)[RTM][RTM], const double* bb
 
 // Gauss-Jordan factors of the SPD R x R matrix held in w (gjf_load_bb;
 // R <= 16 RTM <= 64), computed by one wave (no workgroup barrier).  Step u
-// (pivot rows/columns J = 4u..4u+3, pivot block P) is
-//   W <- W + E_u W(J, :),  E_u(i, :) = -W(i, J) P^-1 (i not in J),
-//                          E_u(J_r, :) = P^-1(r, :) - e_r,
-// i.e. rows J become P^-1 W(J, :) and the others lose W(i, J) P^-1 W(J, :).
-// P^-1 comes from inv4_spd.  The tiles are in the MFMA result layout
-// (element v of lane l: row 16rt + l/16 + 4v, column 16ct + l%16); E_u for
-// row tile rt is stored in the A-operand layout (lane l: row 16rt + l%16,
-// column l/16) at f[(u*4 + rt)*64 + lane], and W(J, ct-tile) is the B
-// operand of the update held by the same lane as element u%4 of tile
+// (pivot rows/columns J = 4u..4u+3, pivot block P = L D L^T, Li = L^-1)
+// is applied in two stages,
+//   W <- W + E1_u W(J, :),  E1_u(J_r, :) = Li(r, :) - e_r (0 off J),
+//   W <- W + E_u W(J, :),   E_u(i, :) = -W(i, J) Li^T D^-1 (i not in J),
+//                           E_u(J_r, :) = (Li^T D^-1)(r, :) - e_r,
+// so that rows J become P^-1 W(J, :) and the others lose
+// (W(i, J) Li^T D^-1)(Li W(J, :)) = W(i, J) P^-1 W(J, :).  This is
+// Gauss-Jordan, but every Schur complement is formed in the symmetric
+// L D L^T association, which keeps the elimination backward stable: with
+// P^-1 formed explicitly the error of each Schur complement grows with
+// cond(P), which breaks the positivity of later pivots once the penalised
+// normal equations reach condition numbers near 1e10.  The tiles are in the
+// MFMA result layout (element v of lane l: row 16rt + l/16 + 4v, column
+// 16ct + l%16); E_u for row tile rt is stored in the A-operand layout (lane
+// l: row 16rt + l%16, column l/16) at f[(u*4 + rt)*64 + lane], E1_u (row
+// tile u/4 only) at f[GJF_E1 + u*64 + lane], and W(J, ct-tile) is the B
+// operand of either stage held by the same lane as element u%4 of tile
 // (u/4, ct), so applying the steps to right-hand sides needs no lane
-// exchange.  f[16*256] = 1 if a pivot is not positive, else 0.  sc: GJF_SC
-// doubles of LDS.  Call with the 64 lanes of one wave.
+// exchange.  f[GJF_FLAG] = 1 if a pivot is not positive, else 0.  sc:
+// GJF_SC doubles of LDS.  Call with the 64 lanes of one wave.
 template <int RTM>
 __device__ __forceinline__ void gj_factor(v4d (&w)[RTM][RTM], int R, double* f,
                                           double* sc) {
@@ -180,7 +179,7 @@ __device__ __forceinline__ void gj_factor(v4d (&w)[RTM][RTM], int R, double* f,
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      double P[4][4], Q[4][4];
+      double P[4][4], Li[4][4], dv[4];
 #pragma unroll
       for (int r = 0; r < 4; r++)
 #pragma unroll
@@ -190,25 +189,38 @@ __device__ __forceinline__ void gj_factor(v4d (&w)[RTM][RTM], int R, double* f,
       for (int rt = 0; rt < RTM; rt++)
 #pragma unroll
         for (int c = 0; c < 4; c++) cb[rt][c] = colB[c * 64 + 16 * rt + lr];
-      inv4_spd(P, Q, bad);
-      // row l/16 of P^-1 (symmetric)
-      double prow[4];
+      ldl4_spd(P, Li, dv, bad);
+      // lrow = row l/16 of Li, lcol = column l/16 of Li, dl = dinv[l/16]
+      double lrow[4], lcol[4];
 #pragma unroll
-      for (int c = 0; c < 4; c++) prow[c] = sel4(Q[0][c], Q[1][c], Q[2][c], Q[3][c], lq);
-      const double pj = sel4(prow[0], prow[1], prow[2], prow[3], lr & 3) -
-                        ((lr & 3) == lq ? 1.0 : 0.0);
+      for (int c = 0; c < 4; c++) {
+        lrow[c] = sel4(Li[0][c], Li[1][c], Li[2][c], Li[3][c], lq);
+        lcol[c] = sel4(Li[c][0], Li[c][1], Li[c][2], Li[c][3], lq);
+      }
+      const double dl = sel4(dv[0], dv[1], dv[2], dv[3], lq);
+      const int r = lr & 3;
+      const double dlt = r == lq ? 1.0 : 0.0;
+      const double e1 = pcol ? sel4(lcol[0], lcol[1], lcol[2], lcol[3], r) - dlt : 0.0;
+      const double pj = sel4(lrow[0], lrow[1], lrow[2], lrow[3], r) * dl - dlt;
       double e[RTM];
 #pragma unroll
       for (int rt = 0; rt < RTM; rt++) {
         double m = 0.0;
 #pragma unroll
-        for (int c = 0; c < 4; c++) m = fma(prow[c], cb[rt][c], m);
-        e[rt] = (rt == jb && pcol) ? pj : -m;
+        for (int c = 0; c < 4; c++) m = fma(lrow[c], cb[rt][c], m);
+        e[rt] = (rt == jb && pcol) ? pj : -dl * m;
         if (rt < RT) f[(u * 4 + rt) * 64 + lane] = e[rt];
       }
+      f[GJF_E1 + u * 64 + lane] = e1;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int ct = 0; ct < RTM; ct++)
+        if (ct >= jb && ct < RT) {
+          const double b1 = w[jb][ct][u4];
+          w[jb][ct] = mfma_f64_4(e1, b1, w[jb][ct]);
+        }
 #pragma unroll
       for (int ct = 0; ct < RTM; ct++) {
         if (ct >= jb && ct < RT) {
@@ -220,7 +232,7 @@ __device__ __forceinline__ void gj_factor(v4d (&w)[RTM][RTM], int R, double* f,
       }
     }
   }
-  if (lane == 0) f[16 * 256] = bad ? 1.0 : 0.0;
+  if (lane == 0) f[GJF_FLAG] = bad ? 1.0 : 0.0;
 }
 
 // 64-bit value of lane l (l uniform)
@@ -916,7 +928,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
     }
   }
   __syncthreads();
-  if (Es[16 * 256] != 0.0) {
+  if (Es[GJF_FLAG] != 0.0) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
@@ -928,6 +940,13 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
       double e[RTM];
 #pragma unroll
       for (int rt = 0; rt < RTM; rt++) e[rt] = Es[(u * 4 + rt) * 64 + lane];
+      const double e1 = Es[GJF_E1 + u * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < NCW; j++)
+        if (u < nst && g - 1 + 3 * j < CB) {   // uniform
+          const double b1 = x[j][jb][u4];
+          x[j][jb] = mfma_f64_4(e1, b1, x[j][jb]);
+        }
 #pragma unroll
       for (int j = 0; j < NCW; j++) {
         if (u < nst && g - 1 + 3 * j < CB) {   // uniform
@@ -1279,14 +1298,17 @@ __device__ __forceinline__ void gs_solve_body(const GsArgs& a, int k, double* T,
   // elimination steps E_u (u < ceil(R/4)) for each row tile, A-operand
   // layout (uniform guards: no waits between the loads)
   const int nst = (R + 3) >> 2;
-  double e[4 * RTM][RTM];
+  double e[4 * RTM][RTM], e1[4 * RTM];
 #pragma unroll
-  for (int u = 0; u < 4 * RTM; u++)
+  for (int u = 0; u < 4 * RTM; u++) {
 #pragma unroll
     for (int rt = 0; rt < RTM; rt++) {
       e[u][rt] = 0.0;
       if (u < nst && rt < RT) e[u][rt] = f[(u * 4 + rt) * 64 + lane];
     }
+    e1[u] = 0.0;
+    if (u < nst) e1[u] = f[GJF_E1 + u * 64 + lane];
+  }
   // Gram tiles of this wave (tile t = g + 4q of the upper triangle,
   // tj <= ti) and their BB entries, loaded with the rest
   const int ntile = RT * (RT + 1) / 2;
@@ -1306,15 +1328,21 @@ __device__ __forceinline__ void gs_solve_body(const GsArgs& a, int k, double* T,
                          t < ntile && ic < R && jr < R);
     }
   }
-  if (f[16 * 256] != 0.0) {
+  if (f[GJF_FLAG] != 0.0) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
-  // X = B after every step W <- W + E_u W(J_u, :); W(J_u, :) of column tile
-  // j is element u%4 of this lane's tile (u/4, j)
+  // X = B after every step W <- W + E1_u W(J_u, :), W <- W + E_u W(J_u, :);
+  // W(J_u, :) of column tile j is element u%4 of this lane's tile (u/4, j)
 #pragma unroll
   for (int u = 0; u < 4 * RTM; u++) {
     const int jb = u >> 2, u4 = u & 3;
+#pragma unroll
+    for (int j = 0; j < NCW; j++)
+      if (u < nst && g + 4 * j < CB) {   // uniform
+        const double b1 = x[j][jb][u4];
+        x[j][jb] = mfma_f64_4(e1[u], b1, x[j][jb]);
+      }
 #pragma unroll
     for (int j = 0; j < NCW; j++) {
       if (u < nst && g + 4 * j < CB) {   // uniform
