@@ -127,6 +127,38 @@ def test_update_phase(hip, name, nb, ranks, ncoup):
         assert e < TOL_T, (i, e)
 
 
+def test_power_and_ortho_ill_conditioned(hip):
+    """Power and Gram-Schmidt phases on the 3-mode Henon-Heiles top block,
+    whose ALS normal equations reach condition numbers near 1e10.  With the
+    Schur complements formed in the L D L^T association both phases agree
+    with the oracle to a few 1e-11 (Gauss-Jordan through explicitly inverted
+    4 x 4 pivot blocks gave 2.3e-10 and 6.0e-10 here); the bound sits
+    between the two."""
+    from mlcp_amd import systems
+    from mlcp_amd.intertwine import Session, block_shift
+    from oracle.orc import C
+    w = systems.hh111()
+    H, Q = w.H, w.Q0.copy()
+    say("power+ortho ill-conditioned", list(Q.nbas), int(Q.ranks.max()), Q.nbloc)
+    es = block_shift(w.Q0.eigv, w.bounds)
+    with Session(Q, H) as s:
+        s.power(w.npow, 1, es)
+        g = s.download()
+    P = Q.copy()
+    for i in range(Q.nbloc):
+        c, b = C.als_pow3(H, *Q.state(i), w.npow, 1, es)
+        e = rel(full_tensor(Q.nbas, *g.state(i)), full_tensor(Q.nbas, c, b))
+        assert e < 1e-10, ("power", i, e)
+        pc, pb = P.state(i)
+        pc[:], pb[:] = c, b
+    with Session(P, H) as s:
+        s.ortho(w.nals)
+        g = s.download()
+    o = C.als_ortho3(P, w.nals)
+    for i, (a, b) in enumerate(zip(tensors(g), tensors(o))):
+        assert rel(a, b) < 1e-10, ("ortho", i, rel(a, b))
+
+
 def test_update_phase_ill_conditioned(hip):
     """The update of the 3-mode Henon-Heiles top block after one power +
     Gram-Schmidt pass: during its ALS sweeps the normal equations reach
