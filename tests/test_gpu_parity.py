@@ -41,6 +41,9 @@ CASES = [
     # ranks above 32 (the four-row-tile factorisation and solve), with
     # bases large enough that the normal equations stay definite
     ("highrank", [16, 14, 18], [40, 20, 48, 33], 6),
+    # the library's limits: rank 64 (four row tiles) and 128 basis functions
+    # per mode (the widest register tiling)
+    ("maxsize", [128, 100, 72], [64, 64, 37, 50], 6),
 ]
 
 
