@@ -101,17 +101,79 @@ def kernel_flops(w, kernel):
     raise ValueError(kernel)
 
 
+def _phase_times_port(w, sub, es):
+    """Phase seconds of one cycle of the C restatement of the reference CPU
+    path (the oracle) on the block ``sub``."""
+    from oracle.orc import C
+    b = sub.nbloc
+    t = {}
+    t0 = time.perf_counter()
+    for s in range(b):
+        C.als_pow3(w.H, *sub.state(s), w.npow, 1, es)
+    t["power"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = C.als_ortho3(sub, w.nals)
+    t["ortho"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for s in range(b):
+        C.prodhv_als3(w.H, *g.state(s), w.nals)
+    t["prodhv"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    C.getoverlaps(g)
+    # S and QHQ: the same pairs of inner products
+    t["overlaps"] = 2 * (time.perf_counter() - t0)
+    U = np.eye(b)
+    t0 = time.perf_counter()
+    for s in range(b):
+        C.sumlcvec3(*g.state(s), g, U[:, s], w.nals)
+    t["update"] = time.perf_counter() - t0
+    return t
+
+
+def _phase_times_reference(w, sub, es):
+    """Phase seconds of one cycle of the compiled reference (its Fortran CPU
+    routines ALS_POW3, ALS_ORTHO_3, PRODHV_ALS3, GetQHQ3 + GetOverlaps and
+    UpdateVecs2, one driver process per phase) on the block ``sub``.  The
+    overlaps' share is GetQHQ3 + GetOverlaps minus PRODHV_ALS3, which
+    GetQHQ3 runs first."""
+    from oracle.orc import C, Ref
+    b = sub.nbloc
+    t = {}
+    t0 = time.perf_counter()
+    Ref.als_pow3(w.H, sub, w.npow, 1, es)
+    t["power"] = time.perf_counter() - t0
+    g = C.als_ortho3(sub, w.nals)     # input of the next phases (bitwise equal)
+    t0 = time.perf_counter()
+    Ref.als_ortho3(w.H, sub, w.nals)
+    t["ortho"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Ref.prodhv_als3(w.H, g, w.nals)
+    t["prodhv"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Ref.getqhq_overlaps(w.H, g, w.nals)
+    t["overlaps"] = max(0.0, time.perf_counter() - t0 - t["prodhv"])
+    t0 = time.perf_counter()
+    Ref.updatevecs2(w.H, g, np.eye(b), w.nals)
+    t["update"] = time.perf_counter() - t0
+    return t
+
+
 def cpu_baseline(w, sample_states, ncycle, min_seconds=10.0):
-    """FP64 oracle (C restatement of the reference CPU path) on one core:
-    each phase of one cycle timed on the first ``b`` states and extrapolated
-    to the whole block by its own scaling (power and reduced H*Q per state;
-    Gram-Schmidt and overlaps per pair; updates per target x source), then
-    times ncycle.  ``b`` starts at ``sample_states`` and doubles until the
-    sample takes ``min_seconds`` or covers the block."""
+    """The reference CPU path on one core: the compiled reference routines
+    (oracle/_ref, built from the reference's Fortran sources at its default
+    -O3) where they are built, else the oracle (the C restatement of the
+    same path, bitwise equal to it).  Each phase of one cycle is timed on
+    the first ``b`` states and extrapolated to the whole block by its own
+    scaling (power and reduced H*Q per state; Gram-Schmidt and overlaps per
+    pair; updates per target x source), then times ncycle.  ``b`` starts at
+    ``sample_states`` and doubles until the sample takes ``min_seconds`` or
+    covers the block."""
     os.environ["OMP_NUM_THREADS"] = "1"
     from mlcp_amd.hoplist import CPBlock
     from mlcp_amd.intertwine import block_shift
-    from oracle.orc import C
+    from oracle.orc import Ref
+    kind = "reference" if Ref.available() else "port"
+    phase_times = _phase_times_reference if kind == "reference" else _phase_times_port
     Q = w.Q0
     B = Q.nbloc
     o = Q.offsets
@@ -120,42 +182,26 @@ def cpu_baseline(w, sample_states, ncycle, min_seconds=10.0):
     while True:
         sub = CPBlock(Q.nbas, Q.ranks[:b], Q.coef[:o[b]], Q.base[:o[b]],
                       Q.eigv[:b])
-        t = {}
-        t0 = time.perf_counter()
-        for s in range(b):
-            C.als_pow3(w.H, *sub.state(s), w.npow, 1, es)
-        t["power"] = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        g = C.als_ortho3(sub, w.nals)
-        t["ortho"] = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        for s in range(b):
-            C.prodhv_als3(w.H, *g.state(s), w.nals)
-        t["prodhv"] = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        C.getoverlaps(g)
-        t["overlaps"] = time.perf_counter() - t0
-        U = np.eye(b)
-        t0 = time.perf_counter()
-        for s in range(b):
-            C.sumlcvec3(*g.state(s), g, U[:, s], w.nals)
-        t["update"] = time.perf_counter() - t0
+        t = phase_times(w, sub, es)
         if sum(t.values()) >= min_seconds or b == B:
             break
         b = min(B, 2 * b)
     scale = {"power": B / b, "prodhv": B / b,
              "ortho": (B * (B - 1) / 2) / max(1, b * (b - 1) / 2),
-             "overlaps": (B * (B + 1)) / (b * (b + 1)) * 2,  # S and QHQ
+             "overlaps": (B * (B + 1)) / (b * (b + 1)),
              "update": (B * B) / (b * b)}
     cycle_s = sum(t[k] * scale[k] for k in t)
     solve_s = cycle_s * ncycle
+    what = ("the compiled reference CPU routines (amdflang -O3)"
+            if kind == "reference" else "the FP64 oracle")
     return {"value": B / solve_s, "unit": "eigenstates/s", "cores": 1,
-            "kind": "port",
-            "sample": (f"one cycle of the FP64 oracle on the first {b} of "
+            "kind": kind,
+            "sample": (f"one cycle of {what} on the first {b} of "
                        f"{B} states ({sum(t.values()):.1f} s), each phase "
                        f"scaled to {B} states by its own cost law, x{ncycle} "
                        f"cycles; every state counts as converged"),
-            "sample_seconds": round(sum(t.values()), 2)}
+            "sample_seconds": round(sum(t.values()), 2),
+            "phase_seconds_per_cycle": {k: round(t[k] * scale[k], 2) for k in t}}
 
 
 def main():
