@@ -158,7 +158,7 @@ def _phase_times_reference(w, sub, es):
     return t
 
 
-def cpu_baseline(w, sample_states, ncycle, min_seconds=10.0):
+def cpu_baseline(w, sample_states, ncycle, min_seconds=20.0):
     """The reference CPU path on one core: the compiled reference routines
     (oracle/_ref, built from the reference's Fortran sources at its default
     -O3) where they are built, else the oracle (the C restatement of the
@@ -216,11 +216,29 @@ def main():
     ap.add_argument("--ncycle", type=int, default=0,
                     help="cycles per step (default: the workload's; fewer "
                          "only for short profiling runs)")
+    ap.add_argument("--only-gs", action="store_true",
+                    help="counter runs: one Gram-Schmidt phase of the "
+                         "workload's initial block (the launch mix of every "
+                         "cycle), kernel by kernel, then exit")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="counter runs: one device-to-device copy of MIB MiB "
+                         "(the counters' calibration), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must "
                           "be 1 (multi-GPU sharding is a design note only)"}))
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        nel = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(nel, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.copy_calibration}), flush=True)
+        return
 
     from mlcp_amd import systems
     from mlcp_amd.intertwine import Session, block_shift
@@ -232,6 +250,14 @@ def main():
     if args.ncycle and args.ncycle != w.ncycle:
         w.ncycle = args.ncycle
         ref, ref_status = None, "not compared (--ncycle)"
+
+    if args.only_gs:
+        os.environ["MLCP_HIP_GRAPH"] = "0"
+        with Session(w.Q0, w.H) as s:
+            s.ortho(w.nals)
+        print(json.dumps({"only_gs": w.name, "gs_steps": int(
+            (w.Q0.nbloc - 1) * w.nals * w.H.ndof)}), flush=True)
+        return
 
     sess = Session(w.Q0, w.H)
     sess.save()
