@@ -44,6 +44,11 @@ CASES = [
     # the library's limits: rank 64 (four row tiles) and 128 basis functions
     # per mode (the widest register tiling)
     ("maxsize", [128, 100, 72], [64, 64, 37, 50], 6),
+    # degenerate shapes: a mode of one basis function and rank-1 states, a
+    # block of one state, two modes (prod_{d != k} G_d is a single Gram)
+    ("edge", [1, 3, 2], [1, 2, 1], 3),
+    ("single", [4, 5, 3], [5], 4),
+    ("twomode", [6, 5], [3, 4, 2], 3),
 ]
 
 
@@ -305,6 +310,24 @@ def test_update_operators(hip, name, nb, ranks, ncoup):
     for o in range(H.nops):
         want = C.update_opermat(Q, int(modes[o]), ops[o])
         assert np.max(np.abs(got[o] - want)) < 1e-12 * max(1.0, np.max(np.abs(want))), o
+
+
+def test_session_rejects_out_of_range_shapes(hip):
+    """Ranks above 64, bases above 128 functions and an empty block are
+    refused with a message before anything is launched."""
+    from mlcp_amd.hoplist import CPBlock
+    from mlcp_amd.intertwine import Session
+    say("session rejects shapes")
+    for nb, ranks, what in (([4, 5], [65], "rank"), ([4, 129], [3], "nbas")):
+        H = random_system(61, nb, ncoup=2)
+        Q = random_block(62, nb, ranks)
+        with pytest.raises(RuntimeError, match=what):
+            Session(Q, H)
+    H = random_system(61, [4, 5], ncoup=2)
+    empty = CPBlock(np.array([4, 5]), np.zeros(0, dtype=np.int32),
+                    np.zeros(0), np.zeros((0, 9)), np.zeros(0))
+    with pytest.raises(RuntimeError, match="nbloc"):
+        Session(empty, H)
 
 
 def test_update_operators_rejects_bad_mode(hip):
