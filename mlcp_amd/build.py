@@ -17,6 +17,11 @@ FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-shared", "-std=c++17",
     # packed FP32 instructions stay off (they are emitted only by hipcc itself)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FP64 MFMA accumulating in VGPRs: the elimination steps read and update
+    # tile elements with vector instructions between MFMAs, and the AGPR form
+    # put eight accumulator copies each way and a full-latency wait around
+    # every MFMA
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall", "-Wno-unused-function",
 ]
 

@@ -1555,7 +1555,7 @@ __global__ void k_lc_weights(double* w, const double* U, int ldU,
 // tiles pass through the wave's LDS so that each column of X is written as
 // runs of consecutive rows.
 template <int RTM>
-__global__ __launch_bounds__(MLCP_NT) void k_lc_refresh(LcArgs a,
+__global__ __launch_bounds__(MLCP_NT, RTM < 4 ? 4 : 2) void k_lc_refresh(LcArgs a,
                                                          const int2* jobs,
                                                          int d) {
   __shared__ double Ts[4][16 * 65];   // per wave: 16 columns x 64 rows (+1)
