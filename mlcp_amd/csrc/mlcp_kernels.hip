@@ -162,7 +162,7 @@ template <int RTM>
 __device__ __forceinline__ void gj_factor(v4d (&w)[RTM][RTM], int R, double* f,
                                           double* sc) {
   const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
-  const int RT = (R + 15) >> 4, nst = (R + 3) >> 2;
+  const int nst = (R + 3) >> 2;
   double* colB = sc;   // [c*64 + row]: the four pivot columns
   bool bad = false;
 #pragma unroll
@@ -172,7 +172,7 @@ __device__ __forceinline__ void gj_factor(v4d (&w)[RTM][RTM], int R, double* f,
       const bool pcol = (lr >> 2) == u4;
 #pragma unroll
       for (int rt = 0; rt < RTM; rt++)
-        if (rt < RT && pcol) {
+        if (pcol) {
 #pragma unroll
           for (int v = 0; v < 4; v++) colB[(lr & 3) * 64 + 16 * rt + lq + 4 * v] = w[rt][jb][v];
         }
@@ -209,7 +209,7 @@ __device__ __forceinline__ void gj_factor(v4d (&w)[RTM][RTM], int R, double* f,
 #pragma unroll
         for (int c = 0; c < 4; c++) m = fma(lrow[c], cb[rt][c], m);
         e[rt] = (rt == jb && pcol) ? pj : -dl * m;
-        if (rt < RT) f[(u * 4 + rt) * 64 + lane] = e[rt];
+        f[(u * 4 + rt) * 64 + lane] = e[rt];
       }
       f[GJF_E1 + u * 64 + lane] = e1;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -217,17 +217,16 @@ __device__ __forceinline__ void gj_factor(v4d (&w)[RTM][RTM], int R, double* f,
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
       for (int ct = 0; ct < RTM; ct++)
-        if (ct >= jb && ct < RT) {
+        if (ct >= jb) {
           const double b1 = w[jb][ct][u4];
           w[jb][ct] = mfma_f64_4(e1, b1, w[jb][ct]);
         }
 #pragma unroll
       for (int ct = 0; ct < RTM; ct++) {
-        if (ct >= jb && ct < RT) {
+        if (ct >= jb) {
           const double bro = w[jb][ct][u4];
 #pragma unroll
-          for (int rt = 0; rt < RTM; rt++)
-            if (rt < RT) w[rt][ct] = mfma_f64_4(e[rt], bro, w[rt][ct]);
+          for (int rt = 0; rt < RTM; rt++) w[rt][ct] = mfma_f64_4(e[rt], bro, w[rt][ct]);
         }
       }
     }
@@ -860,7 +859,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
   const int R = a.R[s], n = a.p.nbas[k];
   const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
   const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
-  const int RT = (R + 15) >> 4, CB = (n + 15) >> 4, nst = (R + 3) >> 2;
+  const int RT = (R + 15) >> 4, nst = (R + 3) >> 2;
   const v4d zero = {0.0, 0.0, 0.0, 0.0};
   const double* BB = a.BB + (long)s * a.rstr;
   if (g == 0) {
@@ -896,8 +895,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
 #pragma unroll
           for (int rt = 0; rt < RTM; rt++) {
             const int l = 16 * rt + lr;
-            av[b][rt] = ld_or0(W + ic + (long)min(l, R - 1) * R,
-                               rt < RT && l < R) * ci;
+            av[b][rt] = ld_or0(W + ic + (long)min(l, R - 1) * R, l < R) * ci;
           }
 #pragma unroll
           for (int j = 0; j < NCW; j++) {
@@ -912,11 +910,9 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
           if (ks0 + b < nks) {
 #pragma unroll
             for (int j = 0; j < NCW; j++)
-              if (g - 1 + 3 * j < CB) {
 #pragma unroll
-                for (int rt = 0; rt < RTM; rt++)
-                  if (rt < RT) x[j][rt] = mfma_f64_4(av[b][rt], bv[b][j], x[j][rt]);
-              }
+              for (int rt = 0; rt < RTM; rt++)
+                x[j][rt] = mfma_f64_4(av[b][rt], bv[b][j], x[j][rt]);
           }
       }
     }
@@ -941,19 +937,17 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
 #pragma unroll
       for (int rt = 0; rt < RTM; rt++) e[rt] = Es[(u * 4 + rt) * 64 + lane];
       const double e1 = Es[GJF_E1 + u * 64 + lane];
+      if (u < nst) {   // uniform
 #pragma unroll
-      for (int j = 0; j < NCW; j++)
-        if (u < nst && g - 1 + 3 * j < CB) {   // uniform
+        for (int j = 0; j < NCW; j++) {
           const double b1 = x[j][jb][u4];
           x[j][jb] = mfma_f64_4(e1, b1, x[j][jb]);
         }
 #pragma unroll
-      for (int j = 0; j < NCW; j++) {
-        if (u < nst && g - 1 + 3 * j < CB) {   // uniform
+        for (int j = 0; j < NCW; j++) {
           const double bro = x[j][jb][u4];
 #pragma unroll
-          for (int rt = 0; rt < RTM; rt++)
-            if (rt < RT) x[j][rt] = mfma_f64_4(e[rt], bro, x[j][rt]);
+          for (int rt = 0; rt < RTM; rt++) x[j][rt] = mfma_f64_4(e[rt], bro, x[j][rt]);
         }
       }
     }
@@ -1275,7 +1269,7 @@ __device__ __forceinline__ void gs_solve_body(const GsArgs& a, int k, double* T,
   const int R = a.Riv, cF = a.cFiv, n = a.p.nbas[k];
   const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
   const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
-  const int RT = (R + 15) >> 4, CB = (n + 15) >> 4;
+  const int RT = (R + 15) >> 4;
   const double* f = a.gjf;
   const double* bjk = a.bjk;
   const v4d zero = {0.0, 0.0, 0.0, 0.0};
@@ -1304,7 +1298,7 @@ __device__ __forceinline__ void gs_solve_body(const GsArgs& a, int k, double* T,
 #pragma unroll
     for (int rt = 0; rt < RTM; rt++) {
       e[u][rt] = 0.0;
-      if (u < nst && rt < RT) e[u][rt] = f[(u * 4 + rt) * 64 + lane];
+      if (u < nst) e[u][rt] = f[(u * 4 + rt) * 64 + lane];
     }
     e1[u] = 0.0;
     if (u < nst) e1[u] = f[GJF_E1 + u * 64 + lane];
@@ -1334,22 +1328,22 @@ __device__ __forceinline__ void gs_solve_body(const GsArgs& a, int k, double* T,
   }
   // X = B after every step W <- W + E1_u W(J_u, :), W <- W + E_u W(J_u, :);
   // W(J_u, :) of column tile j is element u%4 of this lane's tile (u/4, j)
+  // (every tile of the template: padded rows and columns are zero and stay
+  // zero, so only the step count is a branch)
 #pragma unroll
   for (int u = 0; u < 4 * RTM; u++) {
     const int jb = u >> 2, u4 = u & 3;
+    if (u < nst) {   // uniform
 #pragma unroll
-    for (int j = 0; j < NCW; j++)
-      if (u < nst && g + 4 * j < CB) {   // uniform
+      for (int j = 0; j < NCW; j++) {
         const double b1 = x[j][jb][u4];
         x[j][jb] = mfma_f64_4(e1[u], b1, x[j][jb]);
       }
 #pragma unroll
-    for (int j = 0; j < NCW; j++) {
-      if (u < nst && g + 4 * j < CB) {   // uniform
+      for (int j = 0; j < NCW; j++) {
         const double bro = x[j][jb][u4];
 #pragma unroll
-        for (int rt = 0; rt < RTM; rt++)
-          if (rt < RT) x[j][rt] = mfma_f64_4(e[u][rt], bro, x[j][rt]);
+        for (int rt = 0; rt < RTM; rt++) x[j][rt] = mfma_f64_4(e[u][rt], bro, x[j][rt]);
       }
     }
   }

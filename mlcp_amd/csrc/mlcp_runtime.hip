@@ -99,8 +99,9 @@ static int setup_kernels() {
       (const void*)kern<32, m>
   const void* ks[] = {(const void*)k_op_hadamard,  NX_KERNELS(k_op_refresh),
                       NX_KERNELS(k_gs_terms),
-                      NX_KERNELS2(k_gs_reduce, 2), NX_KERNELS2(k_gs_reduce, 3),
-                      NX_KERNELS2(k_gs_reduce, 4), NX_KERNELS2(k_op_solve, 2),
+                      NX_KERNELS2(k_gs_reduce, 1), NX_KERNELS2(k_gs_reduce, 2),
+                      NX_KERNELS2(k_gs_reduce, 3), NX_KERNELS2(k_gs_reduce, 4),
+                      NX_KERNELS2(k_op_solve, 1),  NX_KERNELS2(k_op_solve, 2),
                       NX_KERNELS2(k_op_solve, 3),  NX_KERNELS2(k_op_solve, 4),
                       (const void*)k_lc_solve,     (const void*)k_gep<1>,
                       (const void*)k_gep<2>};
@@ -727,7 +728,10 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
               (size_t)(s->nops + 1) * s->had_tile * sizeof(double), a,
               (const int2*)s->jobs_had.p, k);
       // row tiles of the largest rank of the block
-      if (s->Rmax <= 32)
+      if (s->Rmax <= 16)
+        LAUNCH_NX2(KID_OP_SOLVE, k_op_solve, 1, dim3(s->nbloc), 0, a, k,
+                   ishift, eshift);
+      else if (s->Rmax <= 32)
         LAUNCH_NX2(KID_OP_SOLVE, k_op_solve, 2, dim3(s->nbloc), 0, a, k,
                    ishift, eshift);
       else if (s->Rmax <= 48)
@@ -848,7 +852,9 @@ static int launch_ortho(mlcp_session* s, int nals) {
         const int nred = 1 + (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
         const size_t lds = (size_t)65 * s->nbas[k] * sizeof(double);
         const dim3 gr(nred * g.xst);
-        if (s->ranks[iv] <= 32)
+        if (s->ranks[iv] <= 16)
+          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 1, gr, lds, g, k, nred);
+        else if (s->ranks[iv] <= 32)
           LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 2, gr, lds, g, k, nred);
         else if (s->ranks[iv] <= 48)
           LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 3, gr, lds, g, k, nred);
