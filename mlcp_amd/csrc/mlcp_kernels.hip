@@ -234,211 +234,6 @@ __device__ __forceinline__ void gj_factor(v4d (&w)[RTM][RTM], int R, double* f,
   if (lane == 0) f[GJF_FLAG] = bad ? 1.0 : 0.0;
 }
 
-// 64-bit value of lane l (l uniform)
-__device__ __forceinline__ double readlane_d(double x, int l) {
-  const long long v = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)v, l);
-  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// LDS scratch (doubles) of wg_ldl_solve: unscaled and scaled panel columns
-#define LDL_SC (8 * MLCP_RMAX)
-
-// Solves A X = B for the symmetric positive definite R x R matrix A (lower
-// triangle used, row stride ld, R <= 64, LDS) and n right-hand sides
-// X[l*xs + j] (LDS, overwritten by the solution) by A = L D L^T and
-// substitution, the backward-stable counterpart of the reference's LU solve
-// (getrf/getrs, linalg.cu:73-111): the computed factors are exact for
-// A + dA with |dA| ~ R eps |A|, so the pivots stay positive whenever the
-// penalised normal equations are, however ill-conditioned.  (Pivot blocks
-// inverted explicitly, as in Gauss-Jordan, lose up to eps cond(A) |A| in
-// every Schur complement.)
-//   factor  panels of four columns: wave 0 factors the panel in registers
-//           (lane = row, pivots exchanged by readlane), writes L and D into A
-//           and the unscaled / scaled panel rows into sc; then every wave
-//           applies the rank-4 update A(i,k) -= sum_s W(i,s) W(k,s) / d_s to
-//           its rows of the trailing lower triangle (lane = column);
-//   solve   L y = b and L^T x = D^-1 y by blocks of four rows, one barrier
-//           per block: lane = right-hand side, every wave forms the block's
-//           values and updates its own rows outside the block (four rows in
-//           flight together).
-// Returns false on every thread if a pivot is not positive.
-__device__ bool wg_ldl_solve(double* A, int ld, double* X, int R, int n,
-                             int xs, double* sc, int* flag) {
-  const int lane = TID & 63, g = TID >> 6;
-  double* pn = sc;                 // pn[i*4 + s]: unscaled column s of the panel
-  double* qn = sc + 4 * MLCP_RMAX; // qn[i*4 + s]: L(i, j0 + s)
-  if (TID == 0) *flag = 0;
-  for (int j0 = 0; j0 < R; j0 += 4) {
-    const int w = min(4, R - j0);
-    if (g == 0) {
-      const int i = lane;
-      const bool in = i >= j0 && i < R;
-      const int ic = in ? i : j0;
-      double p[4], rd[4];
-#pragma unroll
-      for (int s = 0; s < 4; s++) p[s] = (in && s < w) ? A[ic * ld + j0 + s] : 0.0;
-#pragma unroll
-      for (int s = 0; s < 4; s++) {
-        if (s < w) {
-          const double d = readlane_d(p[s], j0 + s);
-          rd[s] = 1.0 / d;
-          if (lane == 0 && !(d > 0.0)) *flag = 1;
-#pragma unroll
-          for (int t = s + 1; t < 4; t++)
-            if (t < w) {
-              const double a_ts = readlane_d(p[s], j0 + t);
-              if (i >= j0 + t) p[t] = fma(-p[s] * rd[s], a_ts, p[t]);
-            }
-        } else {
-          rd[s] = 0.0;
-        }
-      }
-#pragma unroll
-      for (int s = 0; s < 4; s++) {
-        const bool below = in && s < w && i > j0 + s;
-        pn[i * 4 + s] = below ? p[s] : 0.0;
-        qn[i * 4 + s] = below ? p[s] * rd[s] : 0.0;
-        if (below) A[i * ld + j0 + s] = p[s] * rd[s];
-        if (in && s < w && i == j0 + s) A[i * ld + i] = p[s];
-      }
-    }
-    __syncthreads();
-    const int r0 = j0 + w;
-    if (r0 < R) {
-      const int k = r0 + lane;
-      const bool kok = k < R;
-      double q[4];
-#pragma unroll
-      for (int s = 0; s < 4; s++) q[s] = kok ? qn[k * 4 + s] : 0.0;
-      for (int i0 = r0 + g; i0 < R; i0 += 16) {
-        double a[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int i = i0 + 4 * u;
-          a[u] = (i < R && kok && k <= i) ? A[i * ld + k] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int ir = min(i0 + 4 * u, R - 1);
-#pragma unroll
-          for (int s = 0; s < 4; s++) a[u] = fma(-pn[ir * 4 + s], q[s], a[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int i = i0 + 4 * u;
-          if (i < R && kok && k <= i) A[i * ld + k] = a[u];
-        }
-      }
-    }
-    __syncthreads();
-  }
-  if (*flag) return false;
-  // substitution: lane = right-hand side, wave g = rows congruent to g mod 4
-  // outside the current block; every wave forms the block's values itself
-  // (the same bits), one wave stores them after the next barrier, when no
-  // wave reads the block's rows any more
-  for (int c0 = 0; c0 < n; c0 += 64) {
-    const int c = c0 + lane;
-    const bool cok = c < n;
-    double* x = X + (cok ? c : 0);
-    double yp[4];
-    int jp = -1, wp = 0;
-    // L y = b, then z = D^-1 y (stored)
-    for (int j0 = 0; j0 < R; j0 += 4) {
-      const int w = min(4, R - j0);
-      __syncthreads();
-      double y[4];
-#pragma unroll
-      for (int s = 0; s < 4; s++) y[s] = s < w ? x[(j0 + s) * xs] : 0.0;
-      if (g == 0 && jp >= 0 && cok) {
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-          if (s < wp) x[(jp + s) * xs] = yp[s] / A[(jp + s) * ld + jp + s];
-      }
-#pragma unroll
-      for (int t = 1; t < 4; t++)
-        if (t < w) {
-#pragma unroll
-          for (int s = 0; s < t; s++) y[t] = fma(-A[(j0 + t) * ld + j0 + s], y[s], y[t]);
-        }
-      for (int i0 = j0 + w + g; i0 < R; i0 += 16) {
-        double v[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) v[u] = i0 + 4 * u < R ? x[(i0 + 4 * u) * xs] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int ir = min(i0 + 4 * u, R - 1);
-#pragma unroll
-          for (int s = 0; s < 4; s++) v[u] = fma(-A[ir * ld + j0 + s], y[s], v[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-          if (i0 + 4 * u < R && cok) x[(i0 + 4 * u) * xs] = v[u];
-      }
-#pragma unroll
-      for (int s = 0; s < 4; s++) yp[s] = y[s];
-      jp = j0;
-      wp = w;
-    }
-    __syncthreads();
-    if (g == 0 && cok) {
-#pragma unroll
-      for (int s = 0; s < 4; s++)
-        if (s < wp) x[(jp + s) * xs] = yp[s] / A[(jp + s) * ld + jp + s];
-    }
-    // L^T x = z
-    jp = -1;
-    for (int j0 = (R - 1) & ~3; j0 >= 0; j0 -= 4) {
-      const int w = min(4, R - j0);
-      __syncthreads();
-      double y[4];
-#pragma unroll
-      for (int s = 0; s < 4; s++) y[s] = s < w ? x[(j0 + s) * xs] : 0.0;
-      if (g == 0 && jp >= 0 && cok) {
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-          if (s < wp) x[(jp + s) * xs] = yp[s];
-      }
-#pragma unroll
-      for (int t = 2; t >= 0; t--)
-#pragma unroll
-        for (int s = t + 1; s < 4; s++)
-          if (s < w) y[t] = fma(-A[(j0 + s) * ld + j0 + t], y[s], y[t]);
-      for (int i0 = g; i0 < j0; i0 += 16) {
-        double v[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) v[u] = i0 + 4 * u < j0 ? x[(i0 + 4 * u) * xs] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int ic = min(i0 + 4 * u, j0 - 1);
-#pragma unroll
-          for (int s = 0; s < 4; s++) {
-            const int jr = min(j0 + s, R - 1);
-            v[u] = fma(-A[jr * ld + ic], y[s], v[u]);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-          if (i0 + 4 * u < j0 && cok) x[(i0 + 4 * u) * xs] = v[u];
-      }
-#pragma unroll
-      for (int s = 0; s < 4; s++) yp[s] = y[s];
-      jp = j0;
-      wp = w;
-    }
-    __syncthreads();
-    if (g == 0 && cok) {
-#pragma unroll
-      for (int s = 0; s < 4; s++)
-        if (s < wp) x[(jp + s) * xs] = yp[s];
-    }
-  }
-  __syncthreads();
-  return true;
-}
-
 // "Construct improved F" (e.g. ALSPow.f90:624-629): coef_l = ||X(l,:)||,
 // F_l^k = X(l,:) / coef_l, written to the pool; X keeps the normalised rows.
 __device__ __forceinline__ void wg_replace_mode(const Pool& p, int k, int c0, int R,
@@ -1722,44 +1517,30 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
   }
 }
 
-// mode-k step of target t (one workgroup): right-hand side = sum of the chunk
-// partials, matrix = prod_{d != k} G_d + pen I, solved by wg_ldl_solve; then
-// the normalised rows replace mode k and its Gram is refreshed
+// mode-k step of target t (one workgroup).  All waves form the matrix
+// prod_{d != k} G_d (the Gram loads in groups of eight) in LDS; then wave 0
+// factors it with the penalty (gj_factor: 4-pivot Gauss-Jordan steps in the
+// L D L^T association) while waves 1-3 sum the chunk partials of the
+// right-hand side (sixteen chunks in flight per thread), 64 elements at a
+// time from a counter in LDS, which wave 0 joins when it has factored; then
+// every wave
+// applies the stored steps to its column tiles of the right-hand side on
+// the FP64 matrix cores (tiles g and g + 4: n <= 128), and the normalised
+// rows replace mode k and refresh its Gram.
+template <int RTM>
 __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
   extern __shared__ double sm[];
-  __shared__ int flag;
-  __shared__ double ldl_sc[LDL_SC];
-  int t = blockIdx.x, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof, xs = n | 1;
-  const int ld = R | 1;
-  double* X = sm;
-  double* A = sm + R * xs;
-  // sum of the chunk partials, sixteen independent chains so the loads of
-  // sixteen chunks are in flight together (the last group from clamped
-  // chunks with weight zero)
-  const double* pt = a.part + (long)t * a.nchunk * a.pstr;
-  for (int e = TID; e < R * n; e += MLCP_NT) {
-    constexpr int U = 16;
-    double sv[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) sv[u] = 0.0;
-    for (int ch = 0; ch < a.nchunk; ch += U) {
-      double pv[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) pv[u] = pt[(long)min(ch + u, a.nchunk - 1) * a.pstr + e];
-#pragma unroll
-      for (int u = 0; u < U; u++) sv[u] += ch + u < a.nchunk ? pv[u] : 0.0;
-    }
-#pragma unroll
-    for (int h = U / 2; h > 0; h >>= 1)
-#pragma unroll
-      for (int u = 0; u < h; u++) sv[u] += sv[u + h];
-    X[(e / n) * xs + e % n] = sv[0];
-  }
-  // matrix: prod_{d != k} G_d + pen I, the Gram loads in groups of eight
+  __shared__ double Es[GJF_SIZE];
+  __shared__ double fsc[GJF_SC];
+  __shared__ int next;
+  const int t = blockIdx.x, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof;
+  const int xs = n | 1;
+  double* X = sm;            // R x n, row stride xs
+  double* A = sm + R * xs;   // R x R, A[i + l R]
+  const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
+  const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
   const double* G = a.G + (long)t * D * a.rstr;
-  double pen = a.pen[t];
   for (int e = TID; e < R * R; e += MLCP_NT) {
-    int i = e % R, l = e / R;
     double bb = 1.0;
     for (int m0 = 1; m0 < D; m0 += 8) {
       double gv[8];
@@ -1772,13 +1553,96 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
       for (int u = 0; u < 8; u++)
         if (m0 + u < D) bb *= gv[u];
     }
-    A[l * ld + i] = bb + (i == l ? pen : 0.0);
+    A[e] = bb;
+  }
+  if (TID == 0) next = 0;
+  __syncthreads();
+  if (g == 0) {
+    v4d w[RTM][RTM];
+    gjf_load_bb<RTM>(w, A, a.pen[t], R);
+    gj_factor<RTM>(w, R, Es, fsc);
+  }
+  {
+    const double* pt = a.part + (long)t * a.nchunk * a.pstr;
+    for (;;) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&next, 64);
+      base = __shfl(base, 0, 64);
+      if (base >= R * n) break;
+      const int e = base + lane;
+      if (e >= R * n) continue;
+      constexpr int U = 16;
+      double sv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) sv[u] = 0.0;
+      for (int ch = 0; ch < a.nchunk; ch += U) {
+        double pv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) pv[u] = pt[(long)min(ch + u, a.nchunk - 1) * a.pstr + e];
+#pragma unroll
+        for (int u = 0; u < U; u++) sv[u] += ch + u < a.nchunk ? pv[u] : 0.0;
+      }
+#pragma unroll
+      for (int h = U / 2; h > 0; h >>= 1)
+#pragma unroll
+        for (int u = 0; u < h; u++) sv[u] += sv[u + h];
+      X[(e / n) * xs + e % n] = sv[0];
+    }
   }
   __syncthreads();
-  if (!wg_ldl_solve(A, ld, X, R, n, xs, ldl_sc, &flag)) {
+  if (Es[GJF_FLAG] != 0.0) {
     if (TID == 0) atomicOr(a.err, 2);
     return;
   }
+  // right-hand-side tiles, element v of lane l: row 16rt + l/16 + 4v,
+  // column 16(g + 4j) + l%16 (zero outside R x n, and stay zero)
+  constexpr int NCW = 2;
+  const int nst = (R + 3) >> 2;
+  v4d x[NCW][RTM];
+#pragma unroll
+  for (int j = 0; j < NCW; j++) {
+    const int m = 16 * (g + 4 * j) + lr;
+#pragma unroll
+    for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = 16 * rt + lq + 4 * v;
+        x[j][rt][v] = (i < R && m < n) ? X[i * xs + m] : 0.0;
+      }
+  }
+#pragma unroll
+  for (int u = 0; u < 4 * RTM; u++) {
+    const int jb = u >> 2, u4 = u & 3;
+    if (u < nst) {   // uniform
+      double e[RTM];
+#pragma unroll
+      for (int rt = 0; rt < RTM; rt++) e[rt] = Es[(u * 4 + rt) * 64 + lane];
+      const double e1 = Es[GJF_E1 + u * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < NCW; j++) {
+        const double b1 = x[j][jb][u4];
+        x[j][jb] = mfma_f64_4(e1, b1, x[j][jb]);
+      }
+#pragma unroll
+      for (int j = 0; j < NCW; j++) {
+        const double bro = x[j][jb][u4];
+#pragma unroll
+        for (int rt = 0; rt < RTM; rt++) x[j][rt] = mfma_f64_4(e[rt], bro, x[j][rt]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NCW; j++) {
+    const int m = 16 * (g + 4 * j) + lr;
+#pragma unroll
+    for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = 16 * rt + lq + 4 * v;
+        if (i < R && m < n) X[i * xs + m] = x[j][rt][v];
+      }
+  }
+  __syncthreads();
   wg_replace_mode(a.p, k, a.col[t], R, X, xs, a.err);
   wg_gram_rows(X, R, n, xs, a.G + ((long)t * D + k) * a.rstr);
 }

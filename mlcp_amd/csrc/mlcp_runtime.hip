@@ -103,7 +103,9 @@ static int setup_kernels() {
                       NX_KERNELS2(k_gs_reduce, 3), NX_KERNELS2(k_gs_reduce, 4),
                       NX_KERNELS2(k_op_solve, 1),  NX_KERNELS2(k_op_solve, 2),
                       NX_KERNELS2(k_op_solve, 3),  NX_KERNELS2(k_op_solve, 4),
-                      (const void*)k_lc_solve,     (const void*)k_gep<1>,
+                      (const void*)k_lc_solve<1>,  (const void*)k_lc_solve<2>,
+                      (const void*)k_lc_solve<3>,  (const void*)k_lc_solve<4>,
+                      (const void*)k_gep<1>,
                       (const void*)k_gep<2>};
 #undef NX_KERNELS2
 #undef NX_KERNELS
@@ -542,7 +544,7 @@ static int check_lds(int Rmax, int nmax, int nbloc) {
    : nx == 16 ? (const void*)kern<16, m>                                    \
    : nx == 24 ? (const void*)kern<24, m> : (const void*)kern<32, m>)
   const bool bad =
-      lds_ok((r * n1 + r * (r | 1)) * b, "k_lc_solve", (const void*)k_lc_solve) ||
+      lds_ok((r * n1 + r * r) * b, "k_lc_solve", (const void*)k_lc_solve<4>) ||
       lds_ok(65 * n * b, "k_gs_reduce", NX_PTR2(k_gs_reduce, 4)) ||
       lds_ok(256 * (size_t)nx * b, "k_gs_terms", NX_PTR(k_gs_terms));
 #undef NX_PTR
@@ -954,8 +956,15 @@ static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
       PLAUNCH(KID_LC_RHS, k_lc_rhs, dim3(nj), dim3(MLCP_NT), 0, a,
               (const int2*)s->lcJobs.p, k);
       size_t lds_s = ((size_t)s->Rmax * (s->nbas[k] | 1) +
-                      (size_t)s->Rmax * (s->Rmax | 1)) * sizeof(double);
-      PLAUNCH(KID_LC_SOLVE, k_lc_solve, dim3(B), dim3(MLCP_NT), lds_s, a, k);
+                      (size_t)s->Rmax * s->Rmax) * sizeof(double);
+      if (s->Rmax <= 16)
+        PLAUNCH(KID_LC_SOLVE, k_lc_solve<1>, dim3(B), dim3(MLCP_NT), lds_s, a, k);
+      else if (s->Rmax <= 32)
+        PLAUNCH(KID_LC_SOLVE, k_lc_solve<2>, dim3(B), dim3(MLCP_NT), lds_s, a, k);
+      else if (s->Rmax <= 48)
+        PLAUNCH(KID_LC_SOLVE, k_lc_solve<3>, dim3(B), dim3(MLCP_NT), lds_s, a, k);
+      else
+        PLAUNCH(KID_LC_SOLVE, k_lc_solve<4>, dim3(B), dim3(MLCP_NT), lds_s, a, k);
       if (!(it == nitn - 1 && k == D - 1)) {
         LC_REFRESH_LAUNCH( dim3(nj), dim3(MLCP_NT), 0, a,
                 (const int2*)s->lcJobs.p, k);
