@@ -1494,25 +1494,28 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
     const int rt = q % RT, ct = q / RT;
     const int i = 16 * rt + lr, mcol = 16 * ct + lr;
     const bool iok = i < R, mok = mcol < n;
-    v4d acc = (v4d){0.0, 0.0, 0.0, 0.0};
+    // operands from clamped rows and columns (finite, weight zero past nc,
+    // R and n), so every load of a batch is in flight together and the
+    // MFMAs need no guard; two accumulators
+    v4d acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    const int ic = min(i, R - 1), mc = min(mcol, n - 1);
     for (int ks0 = 0; ks0 < nks; ks0 += 8) {
       double av[8], bv[8];
 #pragma unroll
       for (int b = 0; b < 8; b++) {
-        const int c = 4 * (ks0 + b) + lq;
-        const bool cok = c < nc;
-        av[b] = (cok && iok) ? P[c * R + i] : 0.0;
-        bv[b] = (cok && mok) ? colp(a.p, k, a.ocol0 + c0 + c)[mcol] : 0.0;
+        const int c = 4 * (ks0 + b) + lq, cc = min(c, nc - 1);
+        av[b] = ld_or0(P + cc * R + ic, c < nc && iok);
+        bv[b] = ld_or0(colp(a.p, k, a.ocol0 + c0 + cc) + mc, c < nc && mok);
       }
 #pragma unroll
-      for (int b = 0; b < 8; b++)
-        if (ks0 + b < nks) acc = mfma_f64_4(av[b], bv[b], acc);
+      for (int b = 0; b < 8; b++) acc[b & 1] = mfma_f64_4(av[b], bv[b], acc[b & 1]);
     }
+    const v4d accs = acc[0] + acc[1];
     // element v of lane l: row 16rt + l/16 + 4v, column 16ct + l%16
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int io = 16 * rt + lq + 4 * v;
-      if (io < R && mok) part[io * n + mcol] = acc[v];
+      if (io < R && mok) part[io * n + mcol] = accs[v];
     }
   }
 }
