@@ -390,7 +390,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
   const int n = a.p.nbas[d], R = a.R[s];
   const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
   const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
-  const int RT = (R + 15) >> 4, NT = (n + 15) >> 4, nks = (n + 3) >> 2;
+  const int RT = (R + 15) >> 4, nks = (n + 3) >> 2;
   if (g >= RT) return;
   const v4d zero = {0.0, 0.0, 0.0, 0.0};
   const int i = 16 * g + lr;                 // this lane's column (term)
@@ -439,11 +439,8 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
         }
 #pragma unroll
         for (int b = 0; b < 8; b++)
-          if (ks0 + b < nks) {
 #pragma unroll
-            for (int mt = 0; mt < MT; mt++)
-              if (mt < NT) y[mt] = mfma_f64_4(av[b][mt], bv[b], y[mt]);
-          }
+          for (int mt = 0; mt < MT; mt++) y[mt] = mfma_f64_4(av[b][mt], bv[b], y[mt]);
       }
     } else {
 #pragma unroll
@@ -1381,13 +1378,11 @@ __global__ __launch_bounds__(MLCP_NT, RTM < 4 ? 4 : 2) void k_lc_refresh(LcArgs 
 #pragma unroll
         for (int rt = 0; rt < RTM; rt++) av[b][rt] = ld_or0(f[rt] + mc, iok[rt] && m < n);
       }
+      // (zero operands past R and n: no guard between the MFMAs)
 #pragma unroll
       for (int b = 0; b < 8; b++)
-        if (ks0 + b < nks) {
 #pragma unroll
-          for (int rt = 0; rt < RTM; rt++)
-            if (rt < RT) acc[rt] = mfma_f64_4(av[b][rt], bv[b], acc[rt]);
-        }
+        for (int rt = 0; rt < RTM; rt++) acc[rt] = mfma_f64_4(av[b][rt], bv[b], acc[rt]);
     }
     // element v of lane l: row 16rt + l/16 + 4v, column 16g + l%16; the
     // tile goes through this wave's LDS so that the stores run along rows
@@ -1430,8 +1425,7 @@ __global__ __launch_bounds__(MLCP_NT, RTM < 4 ? 4 : 2) void k_lc_refresh(LcArgs 
           bv[b2] = ld_or0(fi + mc, ib < R && m < n);
         }
 #pragma unroll
-        for (int b2 = 0; b2 < 8; b2++)
-          if (ks0 + b2 < nks) acc = mfma_f64_4(av[b2], bv[b2], acc);
+        for (int b2 = 0; b2 < 8; b2++) acc = mfma_f64_4(av[b2], bv[b2], acc);
       }
       // element v: l = 16lt + l/16 + 4v, i = 16it + l%16
 #pragma unroll
