@@ -987,7 +987,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
         }
 #pragma unroll
         for (int b = 0; b < 8; b++)
-          if (ks0 + b < nks) acc = mfma_f64_4(av[b], bv[b], acc);
+          acc = mfma_f64_4(av[b], bv[b], acc);   // zero operands past nd
       }
 #pragma unroll
       for (int v = 0; v < 4; v++) {
@@ -1173,7 +1173,7 @@ __device__ __forceinline__ void gs_solve_body(const GsArgs& a, int k, double* T,
         }
 #pragma unroll
         for (int b = 0; b < 8; b++)
-          if (ks0 + b < nks) acc[b & 1] = mfma_f64_4(ta[b], tb[b], acc[b & 1]);
+          acc[b & 1] = mfma_f64_4(ta[b], tb[b], acc[b & 1]);   // zero past n
       }
       gt[q] = acc[0] + acc[1];
       // element v: row j = 16tj + l/16 + 4v, column i = 16ti + l%16
@@ -1706,7 +1706,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_pair_inner(Pool p,
         }
 #pragma unroll
         for (int b = 0; b < 8; b++)
-          if (ks0 + b < nks) c = mfma_f64_4(av[b], bv[b], c);
+          c = mfma_f64_4(av[b], bv[b], c);   // zero operands past n
       }
       prod = prod * c;
     }
