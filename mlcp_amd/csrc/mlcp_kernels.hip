@@ -1303,19 +1303,22 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k,
       a.bjk[e] = sv[0] + a.part[(long)iv * a.pstr + e];
     }
   }
-  // done: results visible device-wide (release), then count; the last
+  // done: results visible device-wide (release: L2 write-back, after the
+  // barrier has made every wave's stores reach L2), then count; the last
   // workgroup resets the counter for the next step and, after an acquire
-  // fence, reads everyone's results
+  // fence (L2 invalidate), reads everyone's results.  The others exit
+  // without invalidating, so workgroups still running on their XCD keep
+  // their cached operands.
   __syncthreads();
   if (TID == 0) {
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const int done = atomicAdd(a.cnt, 1);
     last = (done == nj - 1) ? 1 : 0;
     if (last) atomicExch(a.cnt, 0);
   }
   __syncthreads();
   if (!last) return;
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   gs_solve_body<NX, RTM>(a, k, T, red, cs, rcs);
 }
 
