@@ -1817,6 +1817,9 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
   __syncthreads();
   int m = n + (n & 1);  // players (one dummy when n is odd)
   int np = m / 2;
+  // e / n as a multiply-high by ceil(2^32 / n): exact while e (n - 1) < 2^32,
+  // here e < np n + GEP_U T < 2^15 and n <= 200
+  const unsigned long long ninv = ((1ull << 32) + n - 1) / n;
   for (int sweep = 0; sweep < 60; sweep++) {
     double off = 0.0, tot = 0.0;
     for (int e = tid; e < n * n; e += T) {
@@ -1878,7 +1881,8 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
 #pragma unroll
         for (int u = 0; u < GEP_U; u++) {
           const int e = e0 + u * T + tid;
-          const int q = e < np * n ? e / n : 0, r = e % n;
+          const int qe = (int)(((unsigned long long)e * ninv) >> 32);
+          const int q = e < np * n ? qe : 0, r = e - qe * n;
           const int p0 = pairs[2 * q], p1 = pairs[2 * q + 1];
           live[u] = e < np * n && p1 < n && rs[q] != 0.0;
           o0[u] = live[u] ? r + (long)p0 * n : 0;
@@ -1903,13 +1907,16 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
       for (int e0 = 0; e0 < np * n; e0 += GEP_U * T) {
         double cx[GEP_U], cy[GEP_U], cc[GEP_U], ss[GEP_U];
         long o0[GEP_U], o1[GEP_U];
-        bool live[GEP_U];
+        bool live[GEP_U], zr0[GEP_U], zr1[GEP_U];
 #pragma unroll
         for (int u = 0; u < GEP_U; u++) {
           const int e = e0 + u * T + tid;
-          const int q = e < np * n ? e / n : 0, r = e % n;
+          const int qe = (int)(((unsigned long long)e * ninv) >> 32);
+          const int q = e < np * n ? qe : 0, r = e - qe * n;
           const int p0 = pairs[2 * q], p1 = pairs[2 * q + 1];
           live[u] = e < np * n && p1 < n && rs[q] != 0.0;
+          zr0[u] = r == p1;
+          zr1[u] = r == p0;
           o0[u] = live[u] ? p0 + (long)r * n : 0;
           o1[u] = live[u] ? p1 + (long)r * n : 0;
           cc[u] = rc[q];
@@ -1920,17 +1927,11 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
 #pragma unroll
         for (int u = 0; u < GEP_U; u++)
           if (live[u]) {
-            C[o0[u]] = cc[u] * cx[u] - ss[u] * cy[u];
-            C[o1[u]] = ss[u] * cx[u] + cc[u] * cy[u];
+            // the pair's own entries (p0, p1) and (p1, p0) are annihilated
+            // by the rotation: stored as exact zeros
+            C[o0[u]] = zr0[u] ? 0.0 : cc[u] * cx[u] - ss[u] * cy[u];
+            C[o1[u]] = zr1[u] ? 0.0 : ss[u] * cx[u] + cc[u] * cy[u];
           }
-      }
-      __syncthreads();
-      for (int q = tid; q < np; q += T) {
-        int p0 = pairs[2 * q], p1 = pairs[2 * q + 1];
-        if (p1 < n && rs[q] != 0.0) {
-          C[p0 + (long)p1 * n] = 0.0;
-          C[p1 + (long)p0 * n] = 0.0;
-        }
       }
       __syncthreads();
     }
