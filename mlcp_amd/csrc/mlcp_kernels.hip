@@ -920,14 +920,17 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
   const int lr = lane & 15, lq = lane >> 4;
   const bool rows = 16 * g < Rl;
   const bool icol = lr < ni;
-  // result-layout entries of this lane: (j_v = 16g + lq + 4v, i = lr)
+  // result-layout entries of this lane: (j_v = 16g + lq + 4v, i = lr).  The
+  // own Grams keep G(j, i) at j + i R (the solve reads them); the other
+  // states' cross Grams, read and written only here, are stored i-fastest
+  // at i + j R, so the 16 lanes of a row touch one 128-byte line
   bool ok[4];
   long xo[4];
 #pragma unroll
   for (int v = 0; v < 4; v++) {
     const int j = 16 * g + lq + 4 * v;
     ok[v] = icol && j < Rl;
-    xo[v] = ok[v] ? j + (long)(i0 + lr) * Rl : 0;
+    xo[v] = !ok[v] ? 0 : self ? j + (long)(i0 + lr) * Rl : (i0 + lr) + (long)j * R;
   }
   double cq[4] = {0.0, 0.0, 0.0, 0.0}, qk[4][NT];
   const double cfi = icol ? a.p.coef[cF + i0 + lr] : 0.0;
