@@ -9,7 +9,7 @@ same block).
 
 ALS is a fixed-point iteration whose sweeps amplify a rounding-level
 difference: measured on the MI355X, the 12-mode block's largest eigenvalue
-difference is 1.8e-10 cm^-1 after the first cycle and grows by 30-100x per
+difference is 1.8e-10 cm^-1 after the first cycle and grows by 15-2000x per
 cycle (3.7e-7, 2.3e-5, 8.3e-4, 1.3e-2, ... 5.9 after ten), while the
 Henon-Heiles block stays below 1.4e-5 for all ten.  So the 12-mode block is
 held to per-cycle bounds for its first three cycles only, and the whole
