@@ -33,20 +33,18 @@ __device__ __forceinline__ v4d mfma_f64_4(double a, double b, v4d c) {
 // workgroup helpers
 
 __device__ __forceinline__ double wg_sum(double v, double* red) {
-  // 256-thread sum; red has >= MLCP_NT/64 slots
+  // 256-thread sum, returned to every thread; red has >= MLCP_NT/64 slots and
+  // is used by nothing else (the first barrier keeps a previous call's
+  // readers ahead of this call's writers).  Every thread adds the wave sums
+  // in the same order, so all get the same bits without a broadcast.
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   int w = TID >> 6, l = TID & 63;
   __syncthreads();
   if (l == 0) red[w] = v;
   __syncthreads();
   double s = 0.0;
-  if (TID == 0) {
-    for (int i = 0; i < (MLCP_NT >> 6); i++) s += red[i];
-    red[MLCP_NT >> 6] = s;
-  }
-  __syncthreads();
-  s = red[MLCP_NT >> 6];
-  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < (MLCP_NT >> 6); i++) s += red[i];
   return s;
 }
 
