@@ -3,8 +3,9 @@ the C-ABI, on seeded inputs small enough for the oracle and for full-tensor
 comparison.
 
 Tolerances (FP64 everywhere; the HIP path sums in a different order and
-solves the regularised normal equations by Cholesky where the reference
-uses LU, so agreement is to rounding amplified by a few ALS sweeps):
+solves the regularised normal equations by Gauss-Jordan elimination in the
+L D L^T association where the reference uses LU, so agreement is to
+rounding amplified by a few ALS sweeps):
   represented tensors   ||T_gpu - T_orc|| / ||T_orc|| <= 1e-9
   eigenvalues           |w_gpu - w_orc| <= 1e-9 * max(1, |w|)
 """
