@@ -19,7 +19,10 @@
  * (GetQHQ3 + GetOverlaps + DSYGV) and the vector updates (ALS_SUMLCVEC_3).
  * base/coef are overwritten with the new block, eigv with the eigenvalues
  * (only if update).  On an error the host arrays are left unchanged, the
- * message is printed on stderr and kept for mlcp_hip_last_error().
+ * message is printed on stderr and kept for mlcp_hip_last_error(), and, as
+ * the reference did (throwERROR, utils.cu:111-119), the process exits with
+ * status 1 -- unless MLCP_HIP_ON_ERROR=return is set or the host called
+ * mlcp_hip_set_error_mode(0), in which case the call returns.
  *
  * Extensions (no counterpart in the reference) take plain pointers/sizes and
  * return 0 on success: ragged ranks, a device-resident session for repeated
@@ -58,6 +61,13 @@ void intwdevicemem_(const int *rk, const int *ndof, const int *nbloc,
 
 /* Last error message of the calling thread ("" if none). */
 const char *mlcp_hip_last_error(void);
+
+/* Error behaviour of the void drop-in exports: 1 (the default, also unless
+ * MLCP_HIP_ON_ERROR=return) prints the message and exits with status 1, as
+ * the reference's throwERROR did; 0 returns with the host arrays unchanged
+ * and the message in mlcp_hip_last_error().  No counterpart in the
+ * reference. */
+void mlcp_hip_set_error_mode(int abort_on_error);
 
 /* Same as intertwinecpp_ with per-state ranks (ranks[nbloc]); T terms. */
 int mlcp_hip_intertwine(double *base, double *coef, const int *nbas, int ndof,

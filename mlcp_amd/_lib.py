@@ -42,6 +42,11 @@ def lib():
     L = C.CDLL(LIB_PATH)
     L.mlcp_hip_last_error.restype = C.c_char_p
     L.mlcp_hip_last_error.argtypes = []
+    L.mlcp_hip_set_error_mode.restype = None
+    L.mlcp_hip_set_error_mode.argtypes = [_i]
+    # this host checks mlcp_hip_last_error() after every drop-in call, so a
+    # failed intertwinecpp_/alscpp_ returns instead of ending the process
+    L.mlcp_hip_set_error_mode(0)
     L.intertwinecpp_.restype = None
     L.intertwinecpp_.argtypes = [_dp, _dp, _ip, _pi, _pi, _pi, _ip, _ip, _dp,
                                  _dp, _pi, _pi, _pi, _dp, _pi, _pi, _pi, _pi,

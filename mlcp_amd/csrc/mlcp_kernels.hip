@@ -1736,6 +1736,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_pair_inner(Pool p,
 // in global memory, 1 C in LDS, 2 both in LDS (dynamic, n*n doubles each);
 // the host picks the largest that fits.
 #define GEP_U 4
+#define GEP_SWEEPS 60   // Jacobi sweep limit of k_gep
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
                                               const double* Sin, int n,
@@ -1821,7 +1822,11 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
   // e / n as a multiply-high by ceil(2^32 / n): exact while e (n - 1) < 2^32,
   // here e < np n + GEP_U T < 2^15 and n <= 200
   const unsigned long long ninv = ((1ull << 32) + n - 1) / n;
-  for (int sweep = 0; sweep < 60; sweep++) {
+  // sweep status, set by thread 0 after each off-diagonal norm:
+  // 0 running, 1 converged, 2 non-finite entries
+  __shared__ int jstat;
+  if (tid == 0) jstat = 0;
+  for (int sweep = 0; sweep <= GEP_SWEEPS; sweep++) {
     double off = 0.0, tot = 0.0;
     for (int e = tid; e < n * n; e += T) {
       double v = C[e] * C[e];
@@ -1844,10 +1849,15 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
         a += sred[w];
         b += sred[16 + w];
       }
-      sflag = (a <= 1e-30 * b || a == 0.0) ? 1 : 0;
+      if (!(isfinite(a) && isfinite(b)))
+        jstat = 2;
+      else if (a <= 1e-30 * b || a == 0.0)
+        jstat = 1;
+      else if (sweep == GEP_SWEEPS && a <= 1e-20 * b)
+        jstat = 1;   // rounding-level residue after the last sweep
     }
     __syncthreads();
-    if (sflag) break;
+    if (jstat || sweep == GEP_SWEEPS) break;
     for (int round = 0; round < m - 1; round++) {
       // round-robin pairing: player 0 fixed, others rotate
       for (int q = tid; q < np; q += T) {
@@ -1936,6 +1946,11 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
       }
       __syncthreads();
     }
+  }
+  // not converged or not finite: error bit 8, outputs untouched
+  if (jstat != 1) {
+    if (tid == 0) atomicOr(err, 8);
+    return;
   }
   // ascending order (rank sort, ties by index)
   for (int i = tid; i < n; i += T) {

@@ -37,6 +37,33 @@ static int fail(const std::string& m) {
   return -1;
 }
 
+// Error behaviour of the void drop-in exports (intertwinecpp_, alscpp_),
+// which have no return code.  The reference ends the process on any error
+// (throwERROR, utils.cu:111-119); a host that carried on would copy back
+// its unchanged eigenvalues and read zero change as convergence.  So by
+// default a failed drop-in call prints the message and exits with status 1.
+// MLCP_HIP_ON_ERROR=return, or mlcp_hip_set_error_mode(0), makes the call
+// return instead, for hosts that check mlcp_hip_last_error() themselves.
+static int g_abort_mode = -1;   // -1: not yet read from the environment
+
+extern "C" void mlcp_hip_set_error_mode(int abort_on_error) {
+  g_abort_mode = abort_on_error ? 1 : 0;
+}
+
+static void dropin_end(const char* who, int rc) {
+  if (!rc) return;
+  if (g_abort_mode < 0) {
+    const char* e = getenv("MLCP_HIP_ON_ERROR");
+    g_abort_mode = (e && strcmp(e, "return") == 0) ? 0 : 1;
+  }
+  if (!g_abort_mode) return;
+  fprintf(stderr, "%s: %s\n%s: stopping (set MLCP_HIP_ON_ERROR=return to "
+          "get the error from mlcp_hip_last_error() instead)\n", who,
+          g_err.c_str(), who);
+  fflush(stderr);
+  exit(1);
+}
+
 #define HCHK(x)                                                            \
   do {                                                                     \
     hipError_t e_ = (x);                                                   \
@@ -197,6 +224,9 @@ struct mlcp_session {
   // Gram-Schmidt phase captured as one HIP graph (replayed every cycle)
   hipGraphExec_t gs_exec = nullptr;
   int gs_nals = -1;
+  // a session that runs one cycle and is destroyed (mlcp_hip_intertwine):
+  // capturing the phase would cost more than the single replay saves
+  bool oneshot = false;
   // per-kernel HIP-event probe
   int probe_kid = 0, probe_used = 0;
   std::vector<hipEvent_t> probe_ev;
@@ -609,6 +639,8 @@ static int check_err(mlcp_session* s, const char* phase) {
     if (e & 1) m += "NaN or zero norm on update; ";
     if (e & 2) m += "non-positive pivot in the ALS normal equations; ";
     if (e & 4) m += "overlap matrix not positive definite; ";
+    if (e & 8) m += "block eigensolver: non-finite matrix or no convergence "
+                    "within the sweep limit; ";
     return fail(m);
   }
   return 0;
@@ -805,7 +837,7 @@ static bool use_graphs() {
 static int phase_ortho(mlcp_session* s, int nals) {
   if (nals < 1) return 0;
   bool probing = s->probe_kid == KID_GS_TERMS || s->probe_kid == KID_GS_REDUCE;
-  if (!use_graphs() || probing) return launch_ortho(s, nals);
+  if (!use_graphs() || probing || s->oneshot) return launch_ortho(s, nals);
   if (!s->gs_exec || s->gs_nals != nals) {
     if (s->gs_exec) {
       HCHK(hipGraphExecDestroy(s->gs_exec));
@@ -1505,7 +1537,9 @@ extern "C" int mlcp_hip_gep(int n, const double* A, const double* Bm,
   HCHK(hipGetLastError());
   int err = 0;
   HCHK(hipMemcpy(&err, e.p, sizeof(int), hipMemcpyDeviceToHost));
-  if (err) return fail("gep: overlap matrix not positive definite");
+  if (err & 4) return fail("gep: overlap matrix not positive definite");
+  if (err) return fail("gep: non-finite matrix or no convergence within the "
+                       "sweep limit");
   HCHK(hipMemcpy(X, dX, nn * sizeof(double), hipMemcpyDeviceToHost));
   HCHK(hipMemcpy(w, dw, n * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
@@ -1522,6 +1556,7 @@ extern "C" int mlcp_hip_intertwine(double* base, double* coef,
   if (mlcp_hip_session_create(&s, base, coef, nbas, ndof, nbloc, ranks, opid,
                               mptr, hcoef, mat, T, nops, msize))
     return -1;
+  s->oneshot = true;
   std::vector<double> ev(eigv, eigv + nbloc);
   int rc = run_cycle(s, ev.data(), npow, nals, update, ishift, eshift);
   if (!rc) {
@@ -1549,37 +1584,37 @@ extern "C" void intertwinecpp_(double* base, double* coef, int* nbas,
                                const int* nals, const int* update,
                                const int* ishift, const double* Eshift) {
   (void)ncpu;
+  g_err.clear();
   if (*ndof < 1 || *TxD % *ndof != 0) {
-    fail("TxD must be a multiple of ndof");
+    dropin_end("intertwinecpp_", fail("TxD must be a multiple of ndof"));
     return;
   }
   if (*nbloc < 1 || *nbloc > MLCP_BMAX) {
-    fail("nbloc out of range");
+    dropin_end("intertwinecpp_", fail("nbloc out of range"));
     return;
   }
   std::vector<int> ranks(*nbloc, *rk);
-  mlcp_hip_intertwine(base, coef, nbas, *ndof, *nbloc, ranks.data(), opid,
-                      mptr, Hcoef, mat, *TxD / *ndof, *nops, *msize, eigv,
-                      *npow, *nals, *update != 0, *ishift, *Eshift);
+  dropin_end("intertwinecpp_",
+             mlcp_hip_intertwine(base, coef, nbas, *ndof, *nbloc,
+                                 ranks.data(), opid, mptr, Hcoef, mat,
+                                 *TxD / *ndof, *nops, *msize, eigv, *npow,
+                                 *nals, *update != 0, *ishift, *Eshift));
 }
 
 // alscpp_: F (rF terms) <- ALS fit of G (rG terms), reduc_ALS semantics:
 // penalty 1e-15 max(c_G), no final renormalisation.
-extern "C" void alscpp_(double* coef, double* base, int* nbas,
-                        const int* ndof, const int* rF, const int* rG,
-                        const int* nals) {
+static int alscpp_run(double* coef, double* base, int* nbas, const int* ndof,
+                      const int* rF, const int* rG, const int* nals) {
   g_err.clear();
-  if (*nals < 1) return;
+  if (*nals < 1) return 0;
   int D = *ndof, nF = *rF, nG = *rG;
-  if (nF < 1 || nF > MLCP_RMAX || nG < 1) {
-    fail("alscpp_: rank out of range");
-    return;
-  }
+  if (nF < 1 || nF > MLCP_RMAX || nG < 1)
+    return fail("alscpp_: rank out of range");
   // the block is [F | G]; G is one pseudo-state of rank rG (not limited by
   // MLCP_RMAX since it only appears as old columns)
   int ranks[2] = {nF, std::min(nG, MLCP_RMAX)};
-  if (validate_dims(nbas, D, 1, ranks)) return;
-  if (setup_kernels()) return;
+  if (validate_dims(nbas, D, 1, ranks)) return -1;
+  if (setup_kernels()) return -1;
   mlcp_session s;
   s.ndof = D;
   s.nbloc = 1;
@@ -1592,14 +1627,12 @@ extern "C" void alscpp_(double* coef, double* base, int* nbas,
   s.nD = s.off[D];
   s.Rmax = nF;
   s.nmax = *std::max_element(nbas, nbas + D);
-  if (check_lds(s.Rmax, s.nmax, 1)) return;
-  if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess) {
-    fail("hipStreamCreate failed");
-    return;
-  }
+  if (check_lds(s.Rmax, s.nmax, 1)) return -1;
+  if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess)
+    return fail("hipStreamCreate failed");
   int cap = s.Ntot;
   if (s.fac.need((size_t)cap * s.nD) || s.coef.need(cap) || s.err.need(1))
-    return;
+    return -1;
   s.pool.ndof = D;
   long mo = 0;
   for (int d = 0; d < D; d++) {
@@ -1611,7 +1644,7 @@ extern "C" void alscpp_(double* coef, double* base, int* nbas,
   s.pool.coef = s.coef.p;
   DBuf<double> stage;
   size_t nb = (size_t)s.Ntot * s.nD;
-  if (stage.need(nb + s.Ntot)) return;
+  if (stage.need(nb + s.Ntot)) return -1;
   double pen = 0.0;
   for (int j = 0; j < nG; j++) pen = j ? std::max(pen, coef[nF + j]) : coef[nF];
   pen *= 1.e-15;
@@ -1620,30 +1653,35 @@ extern "C" void alscpp_(double* coef, double* base, int* nbas,
   DBuf<int> dcol, dR;
   if (dcol.put(col, s.st) || dR.put(R, s.st) || s.d_pen.put(pv, s.st) ||
       s.lcW.put(w, s.st))
-    return;
+    return -1;
   if (hipMemcpyAsync(stage.p, base, nb * sizeof(double),
                      hipMemcpyHostToDevice, s.st) != hipSuccess ||
       hipMemcpyAsync(stage.p + nb, coef, s.Ntot * sizeof(double),
-                     hipMemcpyHostToDevice, s.st) != hipSuccess) {
-    fail("alscpp_: upload failed");
-    return;
-  }
+                     hipMemcpyHostToDevice, s.st) != hipSuccess)
+    return fail("alscpp_: upload failed");
   hipLaunchKernelGGL(k_to_pool, dim3(512), dim3(256), 0, s.st, stage.p,
                      stage.p + nb, s.pool, 0, s.Ntot, s.nD);
   if (s.lcX.need((size_t)D * nF * nG) || s.lcG.need((size_t)D * nF * nF) ||
       s.lcPart.need((size_t)((nG + LC_CHUNK - 1) / LC_CHUNK) * nF * s.nmax))
-    return;
+    return -1;
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s.st, s.err.p, 1L, 0);
   LcArgs a = lc_args(&s, 1, dcol.p, dR.p, nF, nF, nG);
-  if (lc_als(&s, a, *nals, false) || check_err(&s, "alscpp_")) return;
+  if (lc_als(&s, a, *nals, false) || check_err(&s, "alscpp_")) return -1;
   std::vector<double> b(nb), c(s.Ntot);
-  if (download_region(&s, 0, b.data(), c.data())) return;
+  if (download_region(&s, 0, b.data(), c.data())) return -1;
   // only F's terms are written back (intertwining.cu:196-197)
   for (int r = 0; r < nF; r++) {
     coef[r] = c[r];
     memcpy(base + (size_t)r * s.nD, b.data() + (size_t)r * s.nD,
            s.nD * sizeof(double));
   }
+  return 0;
+}
+
+extern "C" void alscpp_(double* coef, double* base, int* nbas,
+                        const int* ndof, const int* rF, const int* rG,
+                        const int* nals) {
+  dropin_end("alscpp_", alscpp_run(coef, base, nbas, ndof, rF, rG, nals));
 }
 
 extern "C" void intwdevicemem_(const int* rk, const int* ndof,

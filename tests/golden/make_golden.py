@@ -12,10 +12,11 @@ fixtures:  small seeded inputs and the outputs of the compiled reference
            ReadFFHamiltonian of force-field files and GetHopList of a mode
            Hamiltonian as ffhl_ref.npz.  The C oracle must reproduce them without
            the reference present.
-workloads: the eigenvalues of the FP64 oracle after each cycle of the full
-           top-layer solve of the benchmark workloads (ncycle cycles of
-           GetBlockShift + one intertwine cycle), with a fingerprint of the
-           generated inputs; saved as workload_<name>.json.
+workloads: the spectral bounds (GetSpectralRange) and the eigenvalues of
+           the FP64 oracle after each cycle of the top-layer solve of the
+           benchmark workloads (GetBlockShift + one intertwine cycle per
+           cycle), with a fingerprint of the generated inputs; saved as
+           workload_<name>.json.
 """
 from __future__ import annotations
 
@@ -167,12 +168,38 @@ def ff_and_hoplist():
     print("wrote ffhl")
 
 
-def solve_oracle(w, ncycle=None):
+def spectral_range_oracle(H, rk, npow=10, ncyc=5, seed=1):
+    """GetSpectralRange (Solver.f90:563-654) on the FP64 oracle: the probe
+    vectors of ``spectral_probes`` (the product code's), ALS_POW3 with shift
+    0 for the top and the mid-range shift for the bottom, Rayleigh quotients
+    as the bounds, padded by 0.1% of the range.  The two probes run in two
+    threads (the oracle's calls release the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from mlcp_amd.intertwine import spectral_probes
+    a, b = spectral_probes(H.nbas, rk, seed)
+    lo, hi = C.rayleigh(H, a)[0], C.rayleigh(H, b)[0]
+
+    def step(v, shift):
+        c, bb = C.als_pow3(H, v.coef, v.base, npow, 1, shift)
+        v.coef, v.base = c, bb
+        return C.rayleigh(H, v)[0]
+
+    with ThreadPoolExecutor(2) as ex:
+        for i in range(ncyc):
+            f1 = ex.submit(step, a, 0.5 * (lo + hi))
+            f2 = ex.submit(step, b, 0.0)
+            lo, hi = f1.result(), f2.result()
+            print(f"  spectral range {i + 1}: [{lo:.6f}, {hi:.6f}]", flush=True)
+    pad = 0.001 * (hi - lo)
+    return float(lo - pad), float(hi + pad)
+
+
+def solve_oracle(w, bounds, ncycle, on_cycle=None):
     Q = w.Q0.copy()
     eig = Q.eigv.copy()
     hist, shifts = [], []
-    for c in range(ncycle or w.ncycle):
-        es = block_shift(eig, w.bounds)
+    for c in range(ncycle):
+        es = block_shift(eig, bounds)
         t0 = time.time()
         Q = C.intertwine(w.H, Q, w.npow, w.nals, True, 1, es)
         eig = Q.eigv.copy()
@@ -180,21 +207,43 @@ def solve_oracle(w, ncycle=None):
         shifts.append(float(es))
         print(f"  {w.name} cycle {c + 1}: {time.time() - t0:.1f} s, "
               f"E0 {eig[0]:.6f}", flush=True)
+        if on_cycle:
+            on_cycle(hist, shifts)
     return hist, shifts
 
 
+# cycles of the oracle's trajectory kept per workload (the 24-mode block
+# takes about 25 minutes per oracle cycle on 8 cores)
+GOLDEN_CYCLES = {"hh111": 10, "qff12": 10, "qff24": 3}
+
+
 def workloads(names):
+    """Per workload: the spectral bounds of GetSpectralRange on the oracle
+    (rank min(50, psirank), 10 power sweeps, 5 cycles; Solver.f90:343-348)
+    and the oracle's eigenvalues and E-shifts after each of the first
+    GOLDEN_CYCLES cycles, written after every cycle."""
     makers = {"hh111": systems.hh111, "qff12": systems.qff12,
               "qff24": systems.qff24}
     for n in names:
         w = makers[n]()
-        hist, shifts = solve_oracle(w)
+        path = os.path.join(HERE, f"workload_{n}.json")
+        t0 = time.time()
+        rk = min(50, int(w.Q0.ranks.max()))
+        bounds = spectral_range_oracle(w.H, rk)
+        print(f"  {w.name} spectral range {bounds} "
+              f"({time.time() - t0:.1f} s)", flush=True)
         out = {"workload": w.name, "units": w.units, "ncycle": w.ncycle,
-               "npow": w.npow, "nals": w.nals, "bounds": list(map(float, w.bounds)),
-               "fingerprint": fingerprint(w.H, w.Q0), "eshift": shifts,
-               "eigv": hist}
-        with open(os.path.join(HERE, f"workload_{n}.json"), "w") as f:
-            json.dump(out, f, indent=1)
+               "npow": w.npow, "nals": w.nals, "bounds": list(bounds),
+               "spectral_rank": rk, "fingerprint": fingerprint(w.H, w.Q0),
+               "eshift": [], "eigv": []}
+
+        def write(hist, shifts):
+            out["eshift"], out["eigv"] = shifts, hist
+            with open(path + ".tmp", "w") as f:
+                json.dump(out, f, indent=1)
+            os.replace(path + ".tmp", path)
+
+        solve_oracle(w, bounds, GOLDEN_CYCLES[n], write)
         print("wrote", n)
 
 
@@ -203,6 +252,6 @@ if __name__ == "__main__":
     if what == "fixtures":
         fixtures()
     elif what == "workloads":
-        workloads(sys.argv[2:] or ["hh111", "qff12"])
+        workloads(sys.argv[2:] or ["hh111", "qff12", "qff24"])
     else:
         raise SystemExit(__doc__)
