@@ -153,6 +153,21 @@ int mlcp_hip_session_rayleigh(mlcp_session *s, double *rq);
 int mlcp_hip_gep(int n, const double *A, const double *B, double *X,
                  double *w);
 
+/* The same eigensolver without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  dA, dB (upper triangles, column major
+ * n x n), dX, dw and dinfo are device pointers owned by the caller; the work
+ * matrices live in a fixed device array of the library, one per GPU, sized
+ * for n <= 256 (about 2 MB); larger n is refused with an error.  The call is
+ * asynchronous on `stream` (a hipStream_t; 0 for the null stream) and only
+ * enqueues kernels, so it can be captured.  *dinfo is 0 on success, else
+ * error bits (4: B not positive definite, 8: non-finite or no convergence),
+ * in which case dX and dw are not written.  Calls that overlap on one GPU
+ * share the work array and race: order them on one stream.  Returns 0, or
+ * -1 with mlcp_hip_last_error() set for arguments refused before launch. */
+int mlcp_hip_gep_noalloc(int n, const double *dA, const double *dB,
+                         double *dX, double *dw, int *dinfo, void *stream);
+
 /* Operator update for the layer above the solved one: the GPU form of
  * UpdateH's loop over the primitive operators of the solved mode
  * (Updater.f90:53-107), each operator going through UpdateOperMat

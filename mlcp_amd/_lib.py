@@ -94,6 +94,8 @@ def lib():
     L.mlcp_hip_session_rayleigh.argtypes = [_v, _dp]
     L.mlcp_hip_gep.restype = _i
     L.mlcp_hip_gep.argtypes = [_i, _dp, _dp, _dp, _dp]
+    L.mlcp_hip_gep_noalloc.restype = _i
+    L.mlcp_hip_gep_noalloc.argtypes = [_i, _v, _v, _v, _v, _v, _v]
     L.mlcp_hip_update_opers.restype = _i
     L.mlcp_hip_update_opers.argtypes = [_dp, _dp, _ip, _i, _i, _ip, _i, _ip,
                                         _ip, _dp, _i, _dp]

@@ -1545,6 +1545,64 @@ extern "C" int mlcp_hip_gep(int n, const double* A, const double* Bm,
   return 0;
 }
 
+// --------------------------------------------------------------------------
+// Block eigensolver without allocation, for callers that cannot allocate
+// device memory (code under HIP graph capture, serving loops).  The
+// eigensolver's work matrices live in one file-scope device array of fixed
+// size, one copy per GPU, held until the process ends.  In host code the
+// array's name is only a stand-in: its device address on the current GPU is
+// read once per device with hipGetSymbolAddress and used for every launch.
+#define GEP_NA_NMAX MLCP_BMAX
+#define GEP_NA_WORDS (4 * GEP_NA_NMAX * GEP_NA_NMAX + 2 * GEP_NA_NMAX + 8)
+static __device__ __align__(256) double scratch[GEP_NA_WORDS];
+#define GEP_NA_MAXDEV 64
+static double* g_scratch_dev[GEP_NA_MAXDEV];
+
+static int scratch_ptr(double** out) {
+  int dev = 0;
+  HCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= GEP_NA_MAXDEV)
+    return fail("gep_noalloc: device index out of range");
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    HCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = (double*)ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+extern "C" int mlcp_hip_gep_noalloc(int n, const double* dA, const double* dB,
+                                    double* dX, double* dw, int* dinfo,
+                                    void* stream) {
+  g_err.clear();
+  if (n < 1 || n > GEP_NA_NMAX)
+    return fail("gep_noalloc: n must be in 1.." + std::to_string(GEP_NA_NMAX) +
+                " (the size of its fixed work array)");
+  if (!dA || !dB || !dX || !dw || !dinfo)
+    return fail("gep_noalloc: null device pointer");
+  if (setup_kernels()) return -1;
+  double* ws = nullptr;
+  if (scratch_ptr(&ws)) return -1;
+  hipStream_t st = (hipStream_t)stream;
+  // info is an output: cleared by a fill kernel (capturable), then error
+  // bits are or-ed in by the eigensolver
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, dinfo, 1L, 0);
+  size_t lds = 0;
+  const int mode = gep_mode(n, &lds);
+  if (mode == 2)
+    hipLaunchKernelGGL(k_gep<2>, dim3(1), dim3(1024), lds, st, dA, dB, n, ws,
+                       dX, dw, dinfo);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_gep<1>, dim3(1), dim3(1024), lds, st, dA, dB, n, ws,
+                       dX, dw, dinfo);
+  else
+    hipLaunchKernelGGL(k_gep<0>, dim3(1), dim3(1024), lds, st, dA, dB, n, ws,
+                       dX, dw, dinfo);
+  HCHK(hipGetLastError());
+  return 0;
+}
+
 extern "C" int mlcp_hip_intertwine(double* base, double* coef,
                                    const int* nbas, int ndof, int nbloc,
                                    const int* ranks, const int* opid,
