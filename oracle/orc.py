@@ -25,6 +25,9 @@ from mlcp_amd.hoplist import CPBlock, HopList
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmlcp_oracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+# the same routines built with -fopenmp (states of a block in parallel, as
+# the reference's own build runs them)
+REF_DRIVER_OMP = os.path.join(HERE, "_ref", "ref_driver_omp")
 HOST_DRIVER = os.path.join(HERE, "_ref", "host_driver")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -214,10 +217,13 @@ class Ref:
     """The compiled reference routines, through oracle/_ref/ref_driver."""
 
     MAGIC = 1296843600
+    # driver binary used by run() when none is named (ref_driver, or
+    # REF_DRIVER_OMP for runs with OMP_NUM_THREADS > 1)
+    DRIVER = REF_DRIVER
 
     @staticmethod
-    def available() -> bool:
-        return os.path.exists(REF_DRIVER)
+    def available(omp: bool = False) -> bool:
+        return os.path.exists(REF_DRIVER_OMP if omp else REF_DRIVER)
 
     @staticmethod
     def run(mode, H: HopList, Q: CPBlock, npow=0, nals=0, update=0, ishift=1,
@@ -249,7 +255,7 @@ class Ref:
                     f.write(np.ascontiguousarray(a, dtype=np.int32).tobytes())
                 for a in (H.coef, H.mat, Q.coef, Q.base, Q.eigv, extra):
                     f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
-            p = subprocess.run([driver or REF_DRIVER, fin, fout],
+            p = subprocess.run([driver or Ref.DRIVER, fin, fout],
                                capture_output=True, text=True, timeout=600,
                                cwd=td)
             if p.returncode != 0 or not os.path.exists(fout):

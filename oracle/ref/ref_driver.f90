@@ -181,15 +181,21 @@
            status='replace')
       SELECT CASE (mode)
       CASE (1)
+!        states in parallel, as Iterate does (Solver.f90:509-521); a
+!        comment unless built with -fopenmp (ref_driver_omp)
+!$omp parallel do schedule(dynamic)
          DO s=1,nbloc
             call ALS_POW3(H,Q(s),npow,ishift,eshift)
          ENDDO
+!$omp end parallel do
          call putblock()
       CASE (2)
          allocate(HQ(nbloc))
+!$omp parallel do schedule(dynamic)
          DO s=1,nbloc
             call PRODHV_ALS3(Q(s),HQ(s),H,ishift,eshift,nals)
          ENDDO
+!$omp end parallel do
          write(11) 0
          DO s=1,nbloc
             write(11) HQ(s)%coef

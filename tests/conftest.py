@@ -13,14 +13,6 @@ def pytest_configure(config):
                             "gpu: needs an MI355X (runs the HIP library)")
 
 
-def gpu_available():
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
-
-
 @pytest.fixture(scope="session")
 def hip():
     """The HIP library; GPU tests fail (not skip) if it is missing."""
