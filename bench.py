@@ -17,7 +17,17 @@ A state counts as converged when both hold:
     written by tests/golden/make_teacher.py; "teacher-forced" error).
 The per-eigenvalue teacher-forced errors are printed.
 
-    python bench.py [--steps K] [--warmup W] [--workload qff24]
+Two forms run by default, each timed on its own solves:
+  plain  the reference's algorithm and fixed sweep counts, FP64;
+  fast   Gram-Schmidt and the ALS fits stop at residual tolerances (the
+         fixed counts as the cap), the power sweeps apply the terms on the
+         FP32 matrix cores, and the learned warm start of the reduced H*Q
+         fit is trained (untimed) on pairs from plain cycles of the same
+         Hamiltonian and used only if it beats the plain start on a held-out
+         block.
+`value` is the fast form's; both forms' figures are in `forms`.
+
+    python bench.py [--steps K] [--warmup W] [--workload qff24] [--form both]
 """
 from __future__ import annotations
 
@@ -33,6 +43,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 TOL = 0.01
+GS_TOL = 1e-5      # fast form: Gram-Schmidt residual tolerance
+ALS_TOL = 1e-7     # fast form: fit-residual tolerance of the ALS fits
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -59,10 +71,11 @@ def load_golden(name, w):
     return ref, ("ok" if ok else "fingerprint mismatch")
 
 
-def teacher_forced(w, name, form_env):
+def teacher_forced(w, name, form=None):
     """Per-state |w_gpu - w_oracle| of one cycle from the committed late
-    block (the same block and shift the oracle ran).  None if the fixture is
-    missing or was made for other inputs."""
+    block (the same block and shift the oracle ran), in the given form
+    ((bits, gs_tol, als_tol, network) or None for plain).  None if the
+    fixture is missing or was made for other inputs."""
     path = os.path.join(ROOT, "tests", "golden", f"teacher_{name}.npz")
     if not os.path.exists(path):
         return None, "missing"
@@ -75,8 +88,14 @@ def teacher_forced(w, name, form_env):
     Q = CPBlock(w.Q0.nbas, z["ranks"].astype(np.int32), z["coef"], z["base"],
                 z["eigv_in"].copy())
     with Session(Q, w.H) as s:
+        if form is not None:
+            bits, gt, at, net = form
+            s.set_warmstart(net)
+            s.set_form(bits, gt, at)
         eig = s.cycle(Q.eigv, int(z["npow"]), int(z["nals"]), True, 1,
                       float(z["eshift"]))
+        s.set_form(0)
+        s.set_warmstart(None)
     return np.abs(eig - z["eigv_out"]), "ok"
 
 
@@ -224,6 +243,13 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0,
                     help="threads of the multi-core CPU baseline (default: "
                          "the cores this process may use)")
+    ap.add_argument("--form", default="both",
+                    choices=["plain", "fast", "both"],
+                    help="plain: the reference's fixed sweep counts, FP64; "
+                         "fast: ALS to tolerance + FP32 term application + "
+                         "learned warm start where it pays; both (default)")
+    ap.add_argument("--ml-cycles", type=int, default=12,
+                    help="plain cycles whose pairs train the warm start")
     ap.add_argument("--ncycle", type=int, default=0,
                     help="cycles per step (default: the workload's)")
     ap.add_argument("--only-gs", action="store_true",
@@ -240,21 +266,17 @@ def main():
         sys.exit(2)
 
     if args.copy_calibration:
-        # one device-to-device copy on the HIP runtime the library uses
-        import ctypes
-        from mlcp_amd._lib import lib
-        lib()
-        rt = ctypes.CDLL("libamdhip64.so.7")
-        nb = ctypes.c_size_t(args.copy_calibration * (1 << 20))
-        a, b = ctypes.c_void_p(), ctypes.c_void_p()
-        assert rt.hipMalloc(ctypes.byref(a), nb) == 0
-        assert rt.hipMalloc(ctypes.byref(b), nb) == 0
-        assert rt.hipMemset(a, 1, nb) == 0
-        assert rt.hipDeviceSynchronize() == 0
-        assert rt.hipMemcpy(b, a, nb, 3) == 0      # device to device
-        assert rt.hipDeviceSynchronize() == 0
-        rt.hipFree(a)
-        rt.hipFree(b)
+        # one device-to-device copy by a kernel (torch's elementwise copy),
+        # in a process of its own that never loads the library: torch
+        # brings its own HIP runtime, which must not share a process with
+        # the one the library links
+        import torch
+        nel = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(nel, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
         print(json.dumps({"copy_mib": args.copy_calibration}), flush=True)
         return
 
@@ -262,7 +284,7 @@ def main():
     from mlcp_amd.intertwine import Session, block_shift, spectral_range
 
     makers = {"hh111": systems.hh111, "qff12": systems.qff12,
-              "qff24": systems.qff24}
+              "qff24": systems.qff24, "qff48": systems.qff48}
     print(f"[bench] generating {args.workload}", file=sys.stderr, flush=True)
     w = makers[args.workload]()
     if args.ncycle:
@@ -280,52 +302,106 @@ def main():
     rk = min(50, int(w.Q0.ranks.max()))
     sess = Session(w.Q0, w.H)
     sess.save()
-    phases = np.zeros(5)
-    state = {}
 
-    def solve():
-        sess.restore()
-        bounds = spectral_range(w.H, rk, 10, 5)
-        eig = w.Q0.eigv.copy()
-        prev = eig
-        for _ in range(w.ncycle):
+    def run_form(label, form):
+        """Warmup and timed steps of one form; its convergence count."""
+        phases = np.zeros(5)
+        state = {}
+
+        def solve():
+            sess.restore()
+            bounds = spectral_range(w.H, rk, 10, 5)
+            eig = w.Q0.eigv.copy()
             prev = eig
-            eig = sess.cycle(eig, w.npow, w.nals, True, 1,
-                             block_shift(eig, bounds))
-            phases[:] += sess.phase_ms()
-        state["bounds"] = bounds
-        return eig, prev
+            for _ in range(w.ncycle):
+                prev = eig
+                eig = sess.cycle(eig, w.npow, w.nals, True, 1,
+                                 block_shift(eig, bounds))
+                phases[:] += sess.phase_ms()
+            state["bounds"] = bounds
+            return eig, prev
 
-    for _ in range(args.warmup):
-        solve()
-    phases[:] = 0.0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eig, prev = solve()
-    dt = time.perf_counter() - t0
+        if form is not None:
+            bits, gt, at, net = form
+            sess.set_warmstart(net)
+            sess.set_form(bits, gt, at)
+        for _ in range(args.warmup):
+            solve()
+        phases[:] = 0.0
+        print(f"[bench] {label}: {args.steps} timed solves", file=sys.stderr,
+              flush=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            eig, prev = solve()
+        dt = time.perf_counter() - t0
+        sess.set_form(0)
+        sess.set_warmstart(None)
+        tf, tf_status = teacher_forced(w, args.workload, form)
+        change = np.abs(eig - prev)
+        settled = change <= TOL
+        conv = int(np.sum(settled & (tf <= TOL))) if tf is not None else 0
+        return {
+            "value": round(conv * args.steps / dt, 4),
+            "ms_per_step": round(dt / args.steps * 1e3, 3),
+            "converged": conv, "settled_last_cycle": int(np.sum(settled)),
+            "teacher_forced": tf_status,
+            "teacher_forced_max_err": (float(np.max(tf)) if tf is not None
+                                       else None),
+            "teacher_forced_within_tol": (int(np.sum(tf <= TOL))
+                                          if tf is not None else None),
+            "teacher_forced_err": ([float(f"{x:.3g}") for x in tf]
+                                   if tf is not None else None),
+            "last_cycle_change_median": float(np.median(change)),
+            "all_states_per_s": round(w.Q0.nbloc * args.steps / dt, 4),
+            "phase_ms_per_cycle": {
+                k: round(v / (args.steps * w.ncycle), 3)
+                for k, v in zip(["power", "gram_schmidt", "reduced_hq",
+                                 "gep", "updates"], phases)},
+            "bounds": [round(float(x), 6) for x in state["bounds"]],
+        }
+
+    results = {}
+    if args.form in ("plain", "both"):
+        results["plain"] = run_form("plain", None)
+    fit_info = None
+    if args.form in ("fast", "both"):
+        # the learned warm start: trained (untimed) on a plain solve of the
+        # same Hamiltonian, kept only where it beats the plain start
+        print("[bench] warm-start network: collecting and training",
+              file=sys.stderr, flush=True)
+        from mlcp_amd.warmstart import train_on_solve
+        sess.restore()
+        tb = spectral_range(w.H, rk, 10, 5)
+        t0 = time.perf_counter()
+        fit = train_on_solve(sess, w, tb, args.ml_cycles,
+                             skip=args.ml_cycles // 2)
+        fit_info = {"rows": fit.rows,
+                    "seconds": round(time.perf_counter() - t0, 2),
+                    "loss_first_last": [round(float(fit.losses[0]), 7),
+                                        round(float(fit.losses[-1]), 7)],
+                    "heldout_pred_err": fit.pred_err,
+                    "heldout_identity_err": fit.ident_err,
+                    "hq_sweeps_plain": fit.gain.mean_plain,
+                    "hq_sweeps_warm": fit.gain.mean_warm, "used": fit.used}
+        bits = Session.FORM_TOL | Session.FORM_MIXED | (
+            Session.FORM_ML if fit.used else 0)
+        results["fast"] = run_form(
+            "fast", (bits, GS_TOL, ALS_TOL, fit.net if fit.used else None))
 
     # probe pass (untimed): HIP events around every k_gs_reduce launch of
-    # one cycle from the initial block (kernel-by-kernel launches)
+    # one plain cycle from the initial block (kernel-by-kernel launches)
     probe = "gs_reduce"
     sess.restore()
     sess.probe(probe, 1 << 15)
+    b0 = (results.get("plain") or results["fast"])["bounds"]
     sess.cycle(w.Q0.eigv, w.npow, w.nals, True, 1,
-               block_shift(w.Q0.eigv, state["bounds"]))
+               block_shift(w.Q0.eigv, b0))
     pms, pn = sess.probe_read()
     sess.probe(None)
     sess.close()
 
-    print("[bench] teacher-forced cycle", file=sys.stderr, flush=True)
-    tf, tf_status = teacher_forced(w, args.workload, None)
-    change = np.abs(eig - prev)
-    settled = change <= TOL
-    if tf is not None:
-        agree = tf <= TOL
-        conv = int(np.sum(settled & agree))
-    else:
-        agree = None
-        conv = 0
-    value = conv * args.steps / dt
+    head = results.get("fast") or results["plain"]
+    conv = (results.get("plain") or head)["converged"]
 
     avg_ms = pms / max(pn, 1)
     fl, by = kernel_counts(w, probe)
@@ -358,42 +434,42 @@ def main():
         cpu = cpu_baseline(w, cores, conv)
         print("[bench] CPU baseline, 1 core", file=sys.stderr, flush=True)
         cpu1 = cpu_baseline(w, 1, conv)
+    if "fast" in results:
+        form_name = ("fast: ALS to tolerance (Gram-Schmidt %g, fits %g) + "
+                     "FP32 term application in the power sweeps" %
+                     (GS_TOL, ALS_TOL))
+        form_name += (" + learned warm start" if fit_info["used"] else
+                      " (learned warm start trained, not kept: the plain "
+                      "start was better)")
+    else:
+        form_name = "plain-fp64-batched"
     out = {
         "metric": "converged eigenstates/s at <=0.01 cm^-1 vs FP64",
-        "value": round(value, 4), "unit": "eigenstates/s", "n_gpus": 1,
+        "value": head["value"], "unit": "eigenstates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (seeded force field, exact lower "
-        "layers, random-augmented product-function start)",
+        "dtype": "f64+f32" if "fast" in results else "f64",
+        "data": "synthetic (seeded force field, exact lower layers, "
+                "random-augmented product-function start)",
         "config": {"workload": w.name, "top_modes": int(w.H.ndof),
                    "nbas": int(w.H.nbas[0]), "states": int(w.Q0.nbloc),
                    "ranks": [int(w.Q0.ranks.min()), int(w.Q0.ranks.max())],
                    "terms": int(w.H.T), "ncycle": w.ncycle, "npow": w.npow,
-                   "nals": w.nals, "update": True,
-                   "form": "plain-fp64-batched",
-                   "bounds": [round(float(x), 6) for x in state["bounds"]]},
-        "converged": conv,
-        "settled_last_cycle": int(np.sum(settled)),
-        "teacher_forced": tf_status,
-        "teacher_forced_max_err": (float(np.max(tf)) if tf is not None
-                                   else None),
-        "teacher_forced_within_tol": (int(np.sum(agree)) if tf is not None
-                                      else None),
-        "teacher_forced_err": ([float(f"{x:.3g}") for x in tf]
-                               if tf is not None else None),
-        "last_cycle_change_median": float(np.median(change)),
+                   "nals": w.nals, "update": True, "form": form_name},
+        "converged": head["converged"],
         "golden": golden_status,
-        "all_states_per_s": round(w.Q0.nbloc * args.steps / dt, 4),
-        "phase_ms_per_cycle": {k: round(v / (args.steps * w.ncycle), 3)
-                               for k, v in zip(["power", "gram_schmidt",
-                                                "reduced_hq", "gep",
-                                                "updates"], phases)},
+        "forms": results,
+        "warm_start": fit_info,
         "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_1core": cpu1,
     }
+    if "fast" in results and "plain" in results:
+        out["fast_vs_plain_time"] = round(
+            results["plain"]["ms_per_step"] / results["fast"]["ms_per_step"],
+            3)
     if cpu and cpu["all_states_per_s"] > 0:
         out["gpu_vs_cpu_all_states"] = round(
-            out["all_states_per_s"] / cpu["all_states_per_s"], 1)
+            head["all_states_per_s"] / cpu["all_states_per_s"], 1)
     print(json.dumps(out))
 
 

@@ -52,7 +52,9 @@ void intertwinecpp_(double *base, double *coef, int *nbas, const int *rk,
 void alscpp_(double *coef, double *base, int *nbas, const int *ndof,
              const int *rF, const int *rG, const int *nals);
 
-/* intertwining.cu:31-94.  Prints the device memory this library needs. */
+/* intertwining.cu:31-94.  Prints the device memory this library needs for
+ * such a call (an upper bound: the slots of the term list per mode are
+ * bounded by nops). */
 void intwdevicemem_(const int *rk, const int *ndof, const int *nbloc,
                     const int *TxD, const int *nops, const int *msize,
                     const int *ncpu, const int *update, const int *nbas);
@@ -61,6 +63,10 @@ void intwdevicemem_(const int *rk, const int *ndof, const int *nbloc,
 
 /* Last error message of the calling thread ("" if none). */
 const char *mlcp_hip_last_error(void);
+
+/* Bytes of device memory the library holds at the moment (every live
+ * session and call), for checking intwdevicemem_'s figures. */
+long mlcp_hip_device_bytes(void);
 
 /* Error behaviour of the void drop-in exports: 1 (the default, also unless
  * MLCP_HIP_ON_ERROR=return) prints the message and exits with status 1, as
@@ -89,6 +95,17 @@ int mlcp_hip_session_create(mlcp_session **out, const double *base,
 int mlcp_hip_session_cycle(mlcp_session *s, double *eigv, int npow, int nals,
                            int update, int ishift, double eshift);
 int mlcp_hip_session_download(mlcp_session *s, double *base, double *coef);
+/* ncycle cycles of SolveHPsi's loop (Solver.f90:389-442; update on) with
+ * GetBlockShift (BlockPower.f90:20-45) computed on the device from the
+ * previous cycle's eigenvalues and bounds[2], so no cycle waits on the host:
+ * the first cycle runs eagerly, the others replay it as one captured HIP
+ * graph (MLCP_HIP_GRAPH=0: launched kernel by kernel).  eigv in: the
+ * starting eigenvalues; out: the last cycle's.  hist (ncycle x (nbloc+1),
+ * may be NULL): per cycle the eigenvalues, then the shift it used.  Not
+ * for the learned warm-start form (host-side steps). */
+int mlcp_hip_session_solve(mlcp_session *s, double *eigv,
+                           const double *bounds, int ncycle, int npow,
+                           int nals, double *hist);
 /* Replaces the resident block (same ranks). */
 int mlcp_hip_session_upload(mlcp_session *s, const double *base,
                             const double *coef);
@@ -109,6 +126,60 @@ int mlcp_hip_session_restore(mlcp_session *s);
 int mlcp_hip_session_probe(mlcp_session *s, int kernel_id, int max_launches);
 int mlcp_hip_session_probe_read(mlcp_session *s, double *total_ms,
                                 long *launches);
+
+/* Form of a session's cycle (no counterpart in the reference, which always
+ * runs the fixed sweep counts npow/nals):
+ *   form 0          the reference's fixed sweep counts (default)
+ *   form bit 1      ALS phases to a residual tolerance, the fixed counts as
+ *                   the cap: Gram-Schmidt stops a state at the start of a
+ *                   sweep once its component along the previous states,
+ *                   sqrt(sum_l <Q_l, F>^2) with |F| = 1, is <= gs_tol; the
+ *                   reduced H*Q and the vector updates (ALS fits of a fixed
+ *                   target) stop a state once the squared fit residual
+ *                   changes by <= als_tol |F|^2 between sweeps.  The power
+ *                   sweeps (power iterations) always run npow.
+ *   form bit 2      mixed precision: the power sweeps apply the terms
+ *                   (Y = h F) on the FP32 matrix cores with FP32
+ *                   accumulation; Grams, Hadamard products, solves,
+ *                   Gram-Schmidt, the reduced H*Q, the eigenproblem, the
+ *                   updates and Rayleigh quotients stay FP64.
+ *   form bit 4      learned warm start: the reduced H*Q fit starts from the
+ *                   factors the network set by set_warmstart predicts.
+ * sweeps: out3[0..B) Gram-Schmidt, [B..2B) reduced H*Q, [2B..3B) updates:
+ * the sweeps each state needed in the last phase that stopped at the
+ * tolerance (0: the fixed count ran). */
+int mlcp_hip_session_set_form(mlcp_session *s, int form, double gs_tol,
+                              double als_tol);
+int mlcp_hip_session_sweeps(mlcp_session *s, int *out3);
+
+/* Learned warm start (no counterpart in the reference): a small MLP,
+ * trained and run by the library's own FP32 kernels.  Input: a normalised
+ * factor column (nin values, zero-padded) and a learned nemb-wide embedding
+ * of its mode (nmodes modes); two tanh layers of nhid; output x + W3 h2 + b3
+ * (nin values).  Parameters in one float vector (W1 [nhid][nin+nemb], b1,
+ * W2 [nhid][nhid], b2, W3 [nin][nhid], b3, E [nmodes][nemb]); the output
+ * layer starts at zero, so the untrained network is the identity.
+ *   params      copy the parameters out (set 0) or in (set 1)
+ *   loss_grad   L = 1/(2N) sum |y - t|^2 of N rows and its gradient
+ *   train       Adam (lr, beta 0.9/0.999) for `epochs` passes in shuffled
+ *               minibatches; loss[e] = mean batch loss of pass e
+ * Host arrays: X (N x nin), modes (N), T (N x nin), Y (N x nin). */
+typedef struct mlcp_mlp mlcp_mlp;
+int mlcp_mlp_create(mlcp_mlp **out, int nin, int nemb, int nmodes, int nhid,
+                    unsigned long long seed);
+long mlcp_mlp_nparams(mlcp_mlp *q);
+int mlcp_mlp_params(mlcp_mlp *q, float *p, int set);
+int mlcp_mlp_forward(mlcp_mlp *q, const float *X, const int *modes, int N,
+                     float *Y);
+int mlcp_mlp_loss_grad(mlcp_mlp *q, const float *X, const int *modes,
+                       const float *T, int N, double *loss, float *grad);
+int mlcp_mlp_train(mlcp_mlp *q, const float *X, const int *modes,
+                   const float *T, int N, int epochs, int batch, float lr,
+                   unsigned long long seed, double *loss);
+int mlcp_mlp_destroy(mlcp_mlp *q);
+/* The network a session's form bit 4 uses (NULL: none); it must outlive the
+ * session's use of it and cover the session's modes and basis sizes. */
+int mlcp_hip_session_set_warmstart(mlcp_session *s, mlcp_mlp *q);
 
 /* Launch parameters of a session, the autotuner's knobs (the launch-shape
  * choices of PVVkernel::initnew, modvecvec.cu:27-94, made per problem and

@@ -42,6 +42,8 @@ def lib():
     L = C.CDLL(LIB_PATH)
     L.mlcp_hip_last_error.restype = C.c_char_p
     L.mlcp_hip_last_error.argtypes = []
+    L.mlcp_hip_device_bytes.restype = C.c_long
+    L.mlcp_hip_device_bytes.argtypes = []
     L.mlcp_hip_set_error_mode.restype = None
     L.mlcp_hip_set_error_mode.argtypes = [_i]
     # this host checks mlcp_hip_last_error() after every drop-in call, so a
@@ -64,6 +66,8 @@ def lib():
                                           _ip, _ip, _ip, _dp, _dp, _i, _i, _i]
     L.mlcp_hip_session_cycle.restype = _i
     L.mlcp_hip_session_cycle.argtypes = [_v, _dp, _i, _i, _i, _i, _d]
+    L.mlcp_hip_session_solve.restype = _i
+    L.mlcp_hip_session_solve.argtypes = [_v, _dp, _dp, _i, _i, _i, _dp]
     L.mlcp_hip_session_download.restype = _i
     L.mlcp_hip_session_download.argtypes = [_v, _dp, _dp]
     L.mlcp_hip_session_upload.restype = _i
@@ -99,6 +103,29 @@ def lib():
     L.mlcp_hip_update_opers.restype = _i
     L.mlcp_hip_update_opers.argtypes = [_dp, _dp, _ip, _i, _i, _ip, _i, _ip,
                                         _ip, _dp, _i, _dp]
+    L.mlcp_hip_session_set_form.restype = _i
+    L.mlcp_hip_session_set_form.argtypes = [_v, _i, _d, _d]
+    L.mlcp_hip_session_sweeps.restype = _i
+    L.mlcp_hip_session_sweeps.argtypes = [_v, _ip]
+    _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.mlcp_mlp_create.restype = _i
+    L.mlcp_mlp_create.argtypes = [C.POINTER(_v), _i, _i, _i, _i,
+                                  C.c_ulonglong]
+    L.mlcp_mlp_nparams.restype = C.c_long
+    L.mlcp_mlp_nparams.argtypes = [_v]
+    L.mlcp_mlp_params.restype = _i
+    L.mlcp_mlp_params.argtypes = [_v, _fp, _i]
+    L.mlcp_mlp_destroy.restype = _i
+    L.mlcp_mlp_destroy.argtypes = [_v]
+    L.mlcp_mlp_forward.restype = _i
+    L.mlcp_mlp_forward.argtypes = [_v, _fp, _ip, _i, _fp]
+    L.mlcp_mlp_loss_grad.restype = _i
+    L.mlcp_mlp_loss_grad.argtypes = [_v, _fp, _ip, _fp, _i, _pd, _fp]
+    L.mlcp_mlp_train.restype = _i
+    L.mlcp_mlp_train.argtypes = [_v, _fp, _ip, _fp, _i, _i, _i, C.c_float,
+                                 C.c_ulonglong, _dp]
+    L.mlcp_hip_session_set_warmstart.restype = _i
+    L.mlcp_hip_session_set_warmstart.argtypes = [_v, _v]
     L.mlcp_hip_session_tune_set.restype = _i
     L.mlcp_hip_session_tune_set.argtypes = [_v, _i, _i, _i]
     L.mlcp_hip_session_tune_get.restype = _i

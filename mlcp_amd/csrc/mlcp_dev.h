@@ -79,6 +79,15 @@ struct OpArgs {
   double* BB;            // [B][rstr]
   int maxslots;
   int* err;              // device error word
+  // ALS to a fit-residual tolerance (nullptr: the fixed sweep count):
+  // done[s] = sweeps state s needed once the relative change of |F|^2
+  // between sweeps is at most tol; its later steps exit at once
+  int* done;
+  double* nprev;         // |F|^2 after the previous sweep, per state
+  double tol;
+  // E-shift in device memory (GetBlockShift on the device, so a captured
+  // cycle can be replayed); nullptr: the eshift kernel argument
+  const double* eshift_dev;
 };
 
 // Linear-combination ALS (vector updates, rank reduction), batched over
@@ -101,6 +110,10 @@ struct LcArgs {
   double* part;          // [B][nchunk][Rmax * nmax] partial right-hand sides
   int pstr;              // Rmax * nmax
   int* err;
+  // ALS to a fit-residual tolerance, as in OpArgs (per target)
+  int* done;
+  double* nprev;
+  double tol;
 };
 
 #define LC_CHUNK 64
@@ -128,6 +141,12 @@ struct GsArgs {
   double* gjf;           // [GJF_SIZE] block Gauss-Jordan factors of the step
   int* cnt;              // workgroups of k_gs_reduce done in this step (0 between steps)
   int* err;
+  // Gram-Schmidt to a residual tolerance (nullptr: the reference's fixed
+  // sweep count): done[iv] = 1 once the state's component in the span of
+  // the previous states, sqrt(sum_l <Q_l, F>^2) with |F| = 1, is at most
+  // sqrt(tol2) at the start of a sweep; every later step of the state exits
+  int* done;
+  double tol2;
 };
 // factor tiles (pivot block x row block x 256) and the pivot flag
 // block Gauss-Jordan factors: 16 steps x 4 row tiles of E_u, 16 tiles of

@@ -24,6 +24,7 @@
 //   B (4 x 16):  lane l holds B[l / 16][l % 16]
 //   C, D (16 x 16): lane l, element v holds D[l / 16 + 4 v][l % 16]
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ v4d mfma_f64_4(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -377,13 +378,14 @@ using cptr = const __attribute__((address_space(4))) T*;
 // 4v, column 16g + l%16).  Element ks%4 of tile ks/4 is then exactly the B
 // operand of k-step ks of M = F^T Y (R x R, column tile g), so Y never
 // leaves the registers.  Waves past the state's rank return at once.
-template <int NX>
+template <int NX, int F32>
 __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
                                                          const int2* jobs,
                                                          int do_y) {
   constexpr int MT = NX / 4;   // 16-row tiles of a basis of n <= 4 NX
   const int2 jb = jobs[blockIdx.x];
   const int s = jb.x, op = jb.y;
+  if (a.done && a.done[s]) return;   // converged state (tolerance form)
   const int d = op >= 0 ? a.opmode[op] : -op - 1;
   const int n = a.p.nbas[d], R = a.R[s];
   const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
@@ -420,7 +422,39 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
     const int cls = a.opcls[op];
     const double* h = a.opd + a.opoff[op];
     const double* fo = colp(a.p, d, a.ocol[s] + ic);
-    if (cls == 2) {
+    if (cls == 2 && F32) {
+      // mixed form: the term application on the FP32 matrix cores with FP32
+      // accumulation (v_mfma_f32_16x16x4_f32; its result layout, checked on
+      // the device: element v of lane l is row 4(l/16) + v, column l%16), the
+      // operands rounded to FP32 as they are loaded
+      v4f y32[MT];
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++) y32[mt] = v4f{0.f, 0.f, 0.f, 0.f};
+      for (int ks0 = 0; ks0 < nks; ks0 += 8) {
+        float bv[8], av[8][MT];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+          const int j = 4 * (ks0 + b) + lq, jc = min(j, n - 1);
+          bv[b] = (float)ld_or0(fo + jc, iok && j < n);
+#pragma unroll
+          for (int mt = 0; mt < MT; mt++) {
+            const int m = 16 * mt + lr;
+            av[b][mt] = (float)ld_or0(h + (long)min(m, n - 1) * n + jc,
+                                      m < n && j < n);
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+#pragma unroll
+          for (int mt = 0; mt < MT; mt++)
+            y32[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[b][mt], bv[b],
+                                                           y32[mt], 0, 0, 0);
+      }
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) y[mt][v] = (double)y32[mt][v];
+    } else if (cls == 2) {
       // dense: Y(m, i) = sum_j h[m*n + j] F_old_i[j], k-steps of four j,
       // operands of eight k-steps in flight together
       for (int ks0 = 0; ks0 < nks; ks0 += 8) {
@@ -451,16 +485,21 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
         }
     }
     double* Yg = a.Y + ((long)s * a.nops + op) * a.ystr;
+    const bool l32 = F32 && cls == 2;
     if (iok) {
 #pragma unroll
       for (int mt = 0; mt < MT; mt++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const int m = 16 * mt + lq + 4 * v;
+          const int m = l32 ? 16 * mt + 4 * lq + v : 16 * mt + lq + 4 * v;
           if (m < n) Yg[m + (long)i * n] = y[mt][v];
         }
     }
   }
+  // row of Y that element v of this lane's tile mt holds, within the tile:
+  // lq + 4v (FP64 result layout) or 4 lq + v (FP32 layout); k-step (mt, v)
+  // of the next contraction pairs it with F_l at the same row
+  const bool l32 = F32 && op >= 0 && do_y && a.opcls[op] == 2;
   // M(l, i) = <F_l, Y_i> (or G(l, i) = <F_l, F_i>): row tile lt, this
   // wave's column tile; A operand F_{16lt + l%16}[4ks + l/16]
   double* out = op < 0 ? a.G + ((long)s * a.p.ndof + d) * a.rstr
@@ -474,7 +513,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
     for (int mt = 0; mt < MT; mt++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int m = 16 * mt + 4 * v + lq;
+        const int m = l32 ? 16 * mt + 4 * lq + v : 16 * mt + 4 * v + lq;
         av[mt][v] = ld_or0(fl + min(m, n - 1), la < R && m < n);
       }
     v4d acc = zero;
@@ -482,7 +521,8 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
     for (int mt = 0; mt < MT; mt++)
 #pragma unroll
       for (int v = 0; v < 4; v++)
-        if (4 * mt + v < nks) acc = mfma_f64_4(av[mt][v], y[mt][v], acc);
+        if (l32 ? 16 * mt + v < n : 4 * mt + v < nks)
+          acc = mfma_f64_4(av[mt][v], y[mt][v], acc);
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int l = 16 * lt + lq + 4 * v;
@@ -515,6 +555,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
   // takes elements of group w % G and the slots [hsp[k][p], hsp[k][p+1]) of
   // part p = w / G; all waves of a group share its staged Grams
   const int2 jb = jobs[blockIdx.x];
+  if (a.done && a.done[jb.x]) return;   // converged state (tolerance form)
   const int P = a.nsplit, G = (blockDim.x >> 6) / P, E = 64 * G;
   const int lane = TID & 63, w = __builtin_amdgcn_readfirstlane(TID >> 6);
   const int grp = w % G, part = w / G;
@@ -646,6 +687,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
   __shared__ double fsc[GJF_SC];
   __shared__ double nrm2[4 * 64];
   const int s = blockIdx.x;
+  if (a.done && a.done[s]) return;   // converged state (tolerance form)
   const int R = a.R[s], n = a.p.nbas[k];
   const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
   const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
@@ -667,10 +709,11 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_solve(OpArgs a, int k,
     const int u0 = a.kslot_off[k], u1 = a.kslot_off[k + 1];
     const int nslot = u1 - u0 + (ishift != 0 ? 1 : 0);
     const int nks = (R + 3) >> 2;
+    const double es = a.eshift_dev ? *a.eshift_dev : eshift;
     for (int q = 0; q < nslot; q++) {
       const bool shift = (q == u1 - u0);
       const double* W = shift ? BB : a.W + ((long)s * a.maxslots + q) * a.rstr;
-      const double fac = shift ? -eshift : 1.0;
+      const double fac = shift ? -es : 1.0;
       const double* Y = shift ? nullptr
                               : a.Y + ((long)s * a.nops + a.kslot_op[u0 + q]) * a.ystr;
       // k-steps of four i in batches of KB: all loads of a batch in flight
@@ -808,6 +851,46 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_renorm(OpArgs a) {
   for (int i = TID; i < R; i += MLCP_NT) c[i] *= r;
 }
 
+// c^T (prod_d G_d) c of one state: |F|^2 from its cached own Grams.
+__device__ __forceinline__ double cp_norm2(const double* c, const double* G,
+                                           int rstr, int R, int D,
+                                           double* red) {
+  double acc = 0.0;
+  for (int e = TID; e < R * R; e += MLCP_NT) {
+    int i = e % R, l = e / R;
+    double g = 1.0;
+    for (int d = 0; d < D; d++) g *= G[(long)d * rstr + e];
+    acc += c[i] * c[l] * g;
+  }
+  return wg_sum(acc, red);
+}
+
+// Fit-residual test after ALS sweep `sweep` (0-based) of every state that
+// has not stopped: with the LS fit F of a fixed target T,
+// |F - T|^2 = |T|^2 - <F, T> and <F, T> = |F|^2 at the optimum of the last
+// mode step (up to the 1e-10 penalty), so the change of the squared
+// residual between sweeps is the change of |F|^2.  The state stops (done =
+// sweeps run) once |d|F|^2| <= tol |F|^2.
+__device__ __forceinline__ void conv_test(double nrm, int* done,
+                                          double* nprev, double tol, int s,
+                                          int sweep) {
+  if (TID == 0) {
+    if (sweep > 0 && fabs(nrm - nprev[s]) <= tol * fabs(nrm))
+      done[s] = sweep + 1;
+    nprev[s] = nrm;
+  }
+}
+
+__global__ __launch_bounds__(MLCP_NT) void k_op_conv(OpArgs a, int sweep) {
+  __shared__ double red[8];
+  const int s = blockIdx.x;
+  if (a.done[s]) return;
+  const double nrm = cp_norm2(a.p.coef + a.col[s],
+                              a.G + (long)s * a.p.ndof * a.rstr, a.rstr,
+                              a.R[s], a.p.ndof, red);
+  conv_test(nrm, a.done, a.nprev, a.tol, s, sweep);
+}
+
 // Rayleigh quotients <F, H F> / <F, F> of every state from the operator
 // caches refreshed with Fold = F (RayleighQuotient2, BlockUtils.f90:658-688):
 // <F,HF> = sum_{i,l} c_i c_l sum_t coef_t prod_d M_{op(t,d)}(i,l).
@@ -836,6 +919,33 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_rq(OpArgs a, double* rq) {
   num = wg_sum(num, red);
   den = wg_sum(den, red);
   if (TID == 0) rq[s] = num / den;
+}
+
+// GetBlockShift (BlockPower.f90:20-45) on the device: the shift of the next
+// cycle from the eigenvalues the last one left (ascending), and the bounds.
+__global__ void k_block_shift(const double* eig, int B, double lo, double hi,
+                              double* es) {
+  if (threadIdx.x == 0) {
+    const double a = B > 1 ? 2.0 * eig[B - 1] - eig[B - 2] : 2.0 * eig[0] - lo;
+    *es = 0.5 * (a + hi);
+  }
+}
+
+// Eigenvalues and shift of the cycle numbered *cnt into row *cnt of hist
+// (B + 1 values: the eigenvalues, then the shift), then *cnt += 1; rows past
+// maxc are not written.
+__global__ void k_record(const double* eig, const double* es, int B,
+                         double* hist, int* cnt, int maxc) {
+  __shared__ int c;
+  if (threadIdx.x == 0) c = *cnt;
+  __syncthreads();
+  if (c < maxc) {
+    for (int i = threadIdx.x; i < B; i += blockDim.x)
+      hist[(long)c * (B + 1) + i] = eig[i];
+    if (threadIdx.x == 0) hist[(long)c * (B + 1) + B] = *es;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *cnt = c + 1;
 }
 
 // --------------------------------------------------------------------------
@@ -903,6 +1013,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
   // Workgroups are dealt to the XCDs round robin; with a.xst = 8 only every
   // 8th one works, which keeps all Gram-Schmidt traffic in one XCD's L2.
   if (blockIdx.x % a.xst) return;
+  if (a.done && a.done[a.iv]) return;   // converged state: step skipped
   const int job = blockIdx.x / a.xst;
   const int iv = a.iv, D = a.p.ndof;
   const int R = a.Riv, cF = a.cFiv, n = a.p.nbas[k];
@@ -1247,7 +1358,9 @@ __device__ __forceinline__ void gs_solve_body(const GsArgs& a, int k, double* T,
 // doubles of dynamic LDS for the solve.
 template <int NX, int RTM>
 __global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k,
-                                                        int nj) {
+                                                        int nj, int chk) {
+  // chk > 0: the first step of sweep chk (counted from 0), where the
+  // tolerance is tested
   extern __shared__ double T[];
   __shared__ double cl[MLCP_BMAX];
   __shared__ double fsc[GJF_SC];
@@ -1255,6 +1368,9 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k,
   __shared__ double cs[64], rcs[64];
   __shared__ int last;
   if (blockIdx.x % a.xst) return;
+  // a converged state's remaining steps are skipped (the flag was set by an
+  // earlier launch, so every workgroup of this one sees the same value)
+  if (a.done && a.done[a.iv]) return;
   const int job = blockIdx.x / a.xst;
   const int iv = a.iv, R = a.Riv, n = a.p.nbas[k];
   if (job == 0) {
@@ -1320,6 +1436,24 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_reduce(GsArgs a, int k,
   __syncthreads();
   if (!last) return;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (chk && a.done) {
+    // first step of a sweep: stop if the state's component along the
+    // previous (normalised) states, sqrt(sum_l <Q_l, F>^2) with |F| = 1,
+    // is within the tolerance; F is then its own Gram-Schmidt target to
+    // that accuracy and further sweeps would change it no more
+    const int nch = (R + GS_ICH - 1) / GS_ICH;
+    double s2 = 0.0;
+    for (int l = TID; l < iv; l += MLCP_NT) {
+      double sl = 0.0;
+      for (int c = 0; c < nch; c++) sl += a.pc[(long)l * GS_PCS + c];
+      s2 += sl * sl;
+    }
+    s2 = wg_sum(s2, red);
+    if (s2 <= a.tol2) {
+      if (TID == 0) a.done[iv] = chk;   // sweeps the state needed
+      return;
+    }
+  }
   gs_solve_body<NX, RTM>(a, k, T, red, cs, rcs);
 }
 
@@ -1350,6 +1484,7 @@ __global__ __launch_bounds__(MLCP_NT, RTM < 4 ? 4 : 2) void k_lc_refresh(LcArgs 
                                                          int d) {
   __shared__ double Ts[4][16 * 65];   // per wave: 16 columns x 64 rows (+1)
   int2 jb = jobs[blockIdx.x];
+  if (a.done && a.done[jb.x]) return;   // converged target (tolerance form)
   int t = jb.x, ch = jb.y, R = a.R[t], n = a.p.nbas[d];
   int c0 = ch * LC_CHUNK, nc = min(LC_CHUNK, a.ncold - c0);
   double* X = a.X + ((long)t * a.p.ndof + d) * ((long)a.xr * a.ncold);
@@ -1447,6 +1582,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
                                                      const int2* jobs, int k) {
   __shared__ double P[LC_CHUNK * MLCP_RMAX];   // P(c, i), c-major
   int2 jb = jobs[blockIdx.x];
+  if (a.done && a.done[jb.x]) return;   // converged target (tolerance form)
   int t = jb.x, ch = jb.y, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof;
   int c0 = ch * LC_CHUNK, nc = min(LC_CHUNK, a.ncold - c0);
   const double* w = a.wcol + (long)t * a.ncold;
@@ -1534,7 +1670,9 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
   __shared__ double Es[GJF_SIZE];
   __shared__ double fsc[GJF_SC];
   __shared__ int next;
-  const int t = blockIdx.x, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof;
+  const int t = blockIdx.x;
+  if (a.done && a.done[t]) return;   // converged target (tolerance form)
+  const int R = a.R[t], n = a.p.nbas[k], D = a.p.ndof;
   const int xs = n | 1;
   double* X = sm;            // R x n, row stride xs
   double* A = sm + R * xs;   // R x R, A[i + l R]
@@ -1649,6 +1787,16 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
 }
 
 // final renormalisation (ALSUtils.f90:1030-1039)
+__global__ __launch_bounds__(MLCP_NT) void k_lc_conv(LcArgs a, int sweep) {
+  __shared__ double red[8];
+  const int t = blockIdx.x;
+  if (a.done[t]) return;
+  const double nrm = cp_norm2(a.p.coef + a.col[t],
+                              a.G + (long)t * a.p.ndof * a.rstr, a.rstr,
+                              a.R[t], a.p.ndof, red);
+  conv_test(nrm, a.done, a.nprev, a.tol, t, sweep);
+}
+
 __global__ __launch_bounds__(MLCP_NT) void k_lc_final(LcArgs a) {
   __shared__ double red[8];
   int t = blockIdx.x, R = a.R[t], D = a.p.ndof;

@@ -15,6 +15,7 @@
 //   download block.
 // Everything runs on one non-default stream of default priority.
 #include "mlcp_kernels.hip"
+#include "mlcp_mlp.hip"
 #include <hip/hip_ext.h>
 #include <dlfcn.h>
 #include <fstream>
@@ -30,6 +31,12 @@
 #include "../../include/mlcp_hip.h"
 
 static thread_local std::string g_err;
+
+// forms of the cycle (bits of mlcp_hip_session_set_form)
+#define FORM_TOL 1     // ALS phases to a residual tolerance
+#define FORM_MIXED 2   // power sweeps: term application in FP32
+#define FORM_ML 4      // learned warm start of the reduced H*Q fit
+#define FORM_ALL 7
 
 static int fail(const std::string& m) {
   g_err = m;
@@ -71,19 +78,29 @@ static void dropin_end(const char* who, int rc) {
       return fail(std::string(#x) + ": " + hipGetErrorString(e_));         \
   } while (0)
 
+// Bytes of device memory held by the library's buffers (all sessions).
+static size_t g_dev_bytes = 0;
+
 template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
   int need(size_t cnt) {
     if (cnt <= n) return 0;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
+    release();
     if (cnt == 0) return 0;
     HCHK(hipMalloc((void**)&p, cnt * sizeof(T)));
     n = cnt;
+    g_dev_bytes += cnt * sizeof(T);
     return 0;
+  }
+  void release() {
+    if (p) {
+      (void)hipFree(p);
+      g_dev_bytes -= n * sizeof(T);
+    }
+    p = nullptr;
+    n = 0;
   }
   int put(const std::vector<T>& v, hipStream_t st) {
     if (need(v.size() ? v.size() : 1)) return -1;
@@ -95,9 +112,7 @@ struct DBuf {
     }
     return 0;
   }
-  ~DBuf() {
-    if (p) (void)hipFree(p);
-  }
+  ~DBuf() { release(); }
 };
 
 // Register-tile width NX of the templated kernels for a session: the
@@ -124,7 +139,8 @@ static int setup_kernels() {
   (const void*)kern<8, m>, (const void*)kern<12, m>,                        \
       (const void*)kern<16, m>, (const void*)kern<24, m>,                   \
       (const void*)kern<32, m>
-  const void* ks[] = {(const void*)k_op_hadamard,  NX_KERNELS(k_op_refresh),
+  const void* ks[] = {(const void*)k_op_hadamard,
+                      NX_KERNELS2(k_op_refresh, 0), NX_KERNELS2(k_op_refresh, 1),
                       NX_KERNELS(k_gs_terms),
                       NX_KERNELS2(k_gs_reduce, 1), NX_KERNELS2(k_gs_reduce, 2),
                       NX_KERNELS2(k_gs_reduce, 3), NX_KERNELS2(k_gs_reduce, 4),
@@ -204,6 +220,14 @@ struct mlcp_session {
   // Gram-Schmidt
   DBuf<double> gsX, gsG, gsPart, gsBbs, gsPc, gsBjk, gsGjf;
   DBuf<int> gsCnt;
+  // form of the cycle (mlcp_hip_session_set_form): bit 1 = ALS phases to a
+  // residual tolerance instead of the fixed sweep counts
+  int form = 0;
+  double gs_tol = 0.0, als_tol = 0.0;
+  DBuf<int> gsDone;   // per state: sweeps Gram-Schmidt needed (0: all)
+  struct mlcp_mlp* ws_mlp = nullptr;   // learned warm start (FORM_ML)
+  DBuf<int> opDone, lcDone;     // per state / target: sweeps the reduced
+  DBuf<double> opPrev, lcPrev;  // H*Q and the updates needed; |F|^2
   int had_tile = MLCP_NT;
   // tunables (mlcp_hip_session_tune_set): Hadamard slot ranges per job and
   // the Gram-Schmidt workgroup stride
@@ -213,6 +237,7 @@ struct mlcp_session {
   // linear-combination ALS
   DBuf<double> lcX, lcG, lcPart, lcW;
   DBuf<int2> lcJobs;
+  int lc_jobs_b = -1, lc_jobs_c = -1;   // shape of the uploaded job list
   // GEP
   DBuf<double> QHQ, S, U, eig, ws;
   DBuf<PairJob> pj;
@@ -227,6 +252,15 @@ struct mlcp_session {
   // a session that runs one cycle and is destroyed (mlcp_hip_intertwine):
   // capturing the phase would cost more than the single replay saves
   bool oneshot = false;
+  // device-resident solve (mlcp_hip_session_solve): the power sweeps read
+  // the shift from es_dev (null: the argument); while a cycle is captured
+  // the Gram-Schmidt phase is launched into it rather than replayed
+  const double* es_dev = nullptr;
+  bool capturing = false;
+  DBuf<double> d_es, d_hist;
+  DBuf<int> d_cnt;
+  hipGraphExec_t cyc_exec = nullptr;
+  std::string cyc_key;
   // per-kernel HIP-event probe
   int probe_kid = 0, probe_used = 0;
   std::vector<hipEvent_t> probe_ev;
@@ -234,6 +268,7 @@ struct mlcp_session {
   long probe_n = 0;
   ~mlcp_session() {
     if (gs_exec) (void)hipGraphExecDestroy(gs_exec);
+    if (cyc_exec) (void)hipGraphExecDestroy(cyc_exec);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : probe_ev)
@@ -537,7 +572,9 @@ static int session_alloc(mlcp_session* s) {
   if (s->gsX.need((size_t)B * D * rstr) || s->gsG.need((size_t)D * rstr) ||
       s->gsPart.need((size_t)(B + 1) * pstr) || s->gsBbs.need(rstr) ||
       s->gsPc.need((size_t)(B + 1) * GS_PCS) || s->gsBjk.need(pstr) ||
-      s->gsGjf.need(GJF_SIZE) || s->gsCnt.need(1))
+      s->gsGjf.need(GJF_SIZE) || s->gsCnt.need(1) || s->gsDone.need(B) ||
+      s->opDone.need(B) || s->lcDone.need(B) || s->opPrev.need(B) ||
+      s->lcPrev.need(B))
     return -1;
   // GEP
   std::vector<PairJob> pj;
@@ -626,6 +663,10 @@ static OpArgs op_args(mlcp_session* s, bool work) {
   a.BB = s->BB.p;
   a.maxslots = s->maxslots;
   a.err = s->err.p;
+  a.eshift_dev = s->es_dev;
+  a.done = nullptr;
+  a.nprev = nullptr;
+  a.tol = 0.0;
   return a;
 }
 
@@ -739,9 +780,20 @@ static int probe_collect(mlcp_session* s) {
 // --------------------------------------------------------------------------
 // phases
 
-static int refresh_all(mlcp_session* s, const OpArgs& a, int do_y) {
-  LAUNCH_NX(KID_OP_REFRESH, k_op_refresh, dim3(s->njobs_all), 0, a,
-            (const int2*)s->jobs_all.p, do_y);
+// k_op_refresh with the term application in FP32 (f32 = 1, the mixed form)
+// or FP64
+#define LAUNCH_REFRESH(f32, grid, ...)                                      \
+  do {                                                                    \
+    if (f32)                                                              \
+      LAUNCH_NX2(KID_OP_REFRESH, k_op_refresh, 1, grid, 0, __VA_ARGS__);  \
+    else                                                                  \
+      LAUNCH_NX2(KID_OP_REFRESH, k_op_refresh, 0, grid, 0, __VA_ARGS__);  \
+  } while (0)
+
+static int refresh_all(mlcp_session* s, const OpArgs& a, int do_y,
+                       bool f32 = false) {
+  LAUNCH_REFRESH(f32, dim3(s->njobs_all), a, (const int2*)s->jobs_all.p,
+                 do_y);
   HCHK(hipGetLastError());
   return 0;
 }
@@ -751,10 +803,22 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
                   double eshift, bool renorm, bool do_y_each) {
   OpArgs a = op_args(s, work);
   int D = s->ndof;
+  // the reduced H*Q (a fit of a fixed target) may stop at the fit-residual
+  // tolerance; the power sweeps are power iterations and always run npow
+  if (work && (s->form & FORM_TOL)) {
+    a.done = s->opDone.p;
+    a.nprev = s->opPrev.p;
+    a.tol = s->als_tol;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, s->st, a.done,
+                       (long)s->nbloc, 0);
+  }
   hipLaunchKernelGGL(k_penalty, dim3((s->nbloc + 63) / 64), dim3(64), 0,
                      s->st, s->pool, a.ocol, a.R, 1.e-10, s->d_pen.p,
                      s->nbloc);
-  if (refresh_all(s, a, 1)) return -1;
+  // mixed form: the power sweeps apply the terms in FP32 (the reduced H*Q,
+  // whose fit enters QHQ and so the eigenvalues directly, stays FP64)
+  const bool f32 = !work && (s->form & FORM_MIXED);
+  if (refresh_all(s, a, 1, f32)) return -1;
   for (int it = 0; it < nitn; it++) {
     for (int k = 0; k < D; k++) {
       PLAUNCH(KID_OP_HADAMARD, k_op_hadamard, dim3(s->njobs_had),
@@ -776,12 +840,15 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
                    ishift, eshift);
       bool last = (it == nitn - 1 && k == D - 1);
       if (!last || renorm) {
-        LAUNCH_NX(KID_OP_REFRESH, k_op_refresh, dim3(s->njobs_mode[k]), 0, a,
-                  (const int2*)s->jobs_mode[k].p, do_y_each ? 1 : 0);
+        LAUNCH_REFRESH(f32, dim3(s->njobs_mode[k]), a,
+                       (const int2*)s->jobs_mode[k].p, do_y_each ? 1 : 0);
       }
       if (renorm && k == D - 1)
         hipLaunchKernelGGL(k_op_renorm, dim3(s->nbloc), dim3(MLCP_NT), 0,
                            s->st, a);
+      if (a.done && k == D - 1 && !last)
+        hipLaunchKernelGGL(k_op_conv, dim3(s->nbloc), dim3(MLCP_NT), 0,
+                           s->st, a, it);
     }
   }
   HCHK(hipGetLastError());
@@ -793,11 +860,17 @@ static int phase_power(mlcp_session* s, int npow, int ishift, double eshift) {
   return op_als(s, false, npow, ishift, eshift, true, true);
 }
 
+static int mlp_warm_start(mlcp_session* s, int c0, int ncol);
+
 static int phase_prodhv(mlcp_session* s, int nals) {
   // F <- copy of Fold in the work region, then PRODHV_ALS3 sweeps
   hipLaunchKernelGGL(k_copy_cols, dim3(256), dim3(256), 0, s->st, s->pool, 0,
                      s->Ntot, s->Ntot);
   if (nals < 1) return 0;
+  // learned warm start: the starting factors of the fit predicted from Q's
+  if ((s->form & FORM_ML) && s->ws_mlp &&
+      mlp_warm_start(s, s->Ntot, s->Ntot))
+    return -1;
   return op_als(s, true, nals, 0, 0.0, false, false);
 }
 
@@ -823,6 +896,8 @@ static GsArgs gs_args(mlcp_session* s, int iv) {
   g.gjf = s->gsGjf.p;
   g.cnt = s->gsCnt.p;
   g.err = s->err.p;
+  g.done = (s->form & FORM_TOL) ? s->gsDone.p : nullptr;
+  g.tol2 = s->gs_tol * s->gs_tol;
   return g;
 }
 
@@ -837,7 +912,8 @@ static bool use_graphs() {
 static int phase_ortho(mlcp_session* s, int nals) {
   if (nals < 1) return 0;
   bool probing = s->probe_kid == KID_GS_TERMS || s->probe_kid == KID_GS_REDUCE;
-  if (!use_graphs() || probing || s->oneshot) return launch_ortho(s, nals);
+  if (!use_graphs() || probing || s->oneshot || s->capturing)
+    return launch_ortho(s, nals);
   if (!s->gs_exec || s->gs_nals != nals) {
     if (s->gs_exec) {
       HCHK(hipGraphExecDestroy(s->gs_exec));
@@ -866,6 +942,8 @@ static int launch_ortho(mlcp_session* s, int nals) {
   // the done-counter of k_gs_reduce starts every phase at zero (a fill
   // kernel, so the phase can be captured)
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s->st, s->gsCnt.p, 1L, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, s->st, s->gsDone.p,
+                     (long)s->nbloc, 0);
   hipLaunchKernelGGL(k_normcoef, dim3(1), dim3(MLCP_NT), 0, s->st, s->pool,
                      s->roff[0], s->ranks[0]);
   hipLaunchKernelGGL(k_penalty, dim3((s->nbloc + 63) / 64), dim3(64), 0,
@@ -886,14 +964,16 @@ static int launch_ortho(mlcp_session* s, int nals) {
         const int nred = 1 + (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
         const size_t lds = (size_t)65 * s->nbas[k] * sizeof(double);
         const dim3 gr(nred * g.xst);
+        // tolerance test at the first step of every sweep after the first
+        const int chk = (g.done && it > 0 && k == 0) ? it : 0;
         if (s->ranks[iv] <= 16)
-          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 1, gr, lds, g, k, nred);
+          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 1, gr, lds, g, k, nred, chk);
         else if (s->ranks[iv] <= 32)
-          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 2, gr, lds, g, k, nred);
+          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 2, gr, lds, g, k, nred, chk);
         else if (s->ranks[iv] <= 48)
-          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 3, gr, lds, g, k, nred);
+          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 3, gr, lds, g, k, nred, chk);
         else
-          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 4, gr, lds, g, k, nred);
+          LAUNCH_NX2(KID_GS_REDUCE, k_gs_reduce, 4, gr, lds, g, k, nred, chk);
         kprev = k;
       }
   }
@@ -956,6 +1036,9 @@ static LcArgs lc_args(mlcp_session* s, int B, const int* col, const int* R,
   a.part = s->lcPart.p;
   a.pstr = s->Rmax * s->nmax;
   a.err = s->err.p;
+  a.done = nullptr;
+  a.nprev = nullptr;
+  a.tol = 0.0;
   return a;
 }
 
@@ -974,11 +1057,17 @@ static LcArgs lc_args(mlcp_session* s, int B, const int* col, const int* R,
 // penalties in d_pen).
 static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
   int D = s->ndof, B = a.B;
-  std::vector<int2> jobs;
-  for (int t = 0; t < B; t++)
-    for (int c = 0; c < a.nchunk; c++) jobs.push_back(make_int2(t, c));
-  if (s->lcJobs.put(jobs, s->st)) return -1;
-  int nj = (int)jobs.size();
+  // job list (target, chunk of old columns), uploaded when it changes (so a
+  // captured cycle holds no host copy)
+  const int nj = B * a.nchunk;
+  if (s->lc_jobs_b != B || s->lc_jobs_c != a.nchunk) {
+    std::vector<int2> jobs;
+    for (int t = 0; t < B; t++)
+      for (int c = 0; c < a.nchunk; c++) jobs.push_back(make_int2(t, c));
+    if (s->lcJobs.put(jobs, s->st)) return -1;
+    s->lc_jobs_b = B;
+    s->lc_jobs_c = a.nchunk;
+  }
   for (int d = 0; d < D; d++) {
     LC_REFRESH_LAUNCH( dim3(nj), dim3(MLCP_NT), 0, a,
             (const int2*)s->lcJobs.p, d);
@@ -1000,6 +1089,9 @@ static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
       if (!(it == nitn - 1 && k == D - 1)) {
         LC_REFRESH_LAUNCH( dim3(nj), dim3(MLCP_NT), 0, a,
                 (const int2*)s->lcJobs.p, k);
+        if (a.done && k == D - 1)
+          hipLaunchKernelGGL(k_lc_conv, dim3(B), dim3(MLCP_NT), 0, s->st, a,
+                             it);
       }
     }
   if (final_norm)
@@ -1027,6 +1119,13 @@ static int phase_update(mlcp_session* s, const double* Ud, int nals) {
   hipLaunchKernelGGL(k_penalty, dim3((B + 63) / 64), dim3(64), 0, s->st,
                      s->pool, s->d_colQ.p, s->d_R.p, 1.e-10, s->d_pen.p, B);
   LcArgs a = lc_args(s, B, s->d_colQ.p, s->d_R.p, s->Rmax, 2 * N, N);
+  if (s->form & FORM_TOL) {
+    a.done = s->lcDone.p;
+    a.nprev = s->lcPrev.p;
+    a.tol = s->als_tol;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, s->st, a.done,
+                       (long)B, 0);
+  }
   return lc_als(s, a, nals, true);
 }
 
@@ -1056,6 +1155,409 @@ static int run_cycle(mlcp_session* s, double* eigv, int npow, int nals,
   if (update)
     HCHK(hipMemcpy(eigv, s->eig.p, sizeof(double) * s->nbloc,
                    hipMemcpyDeviceToHost));
+  return 0;
+}
+
+
+// --------------------------------------------------------------------------
+// Learned warm start: the MLP of mlcp_mlp.hip, its training and inference.
+
+struct mlcp_mlp {
+  MlpDims m;
+  hipStream_t st = nullptr;
+  DBuf<float> P, G, Mo, Vo;          // parameters, gradients, Adam moments
+  DBuf<float> X0, Z1, Z2, Y, dY, dH2, dH1, dX0;   // batch work (nbmax rows)
+  DBuf<double> sse;
+  int nbmax = 0;
+  int step = 0;                      // Adam steps taken
+  // training set (device) and warm-start inputs
+  DBuf<float> tX, tT, wX, wY;
+  DBuf<int> tM, tIdx, wM;
+  ~mlcp_mlp() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+static int mlp_batch_bufs(mlcp_mlp* q, int nb) {
+  if (nb <= q->nbmax) return 0;
+  const MlpDims& m = q->m;
+  if (q->X0.need((size_t)nb * m.nx()) || q->Z1.need((size_t)nb * m.nhid) ||
+      q->Z2.need((size_t)nb * m.nhid) || q->Y.need((size_t)nb * m.nout) ||
+      q->dY.need((size_t)nb * m.nout) || q->dH2.need((size_t)nb * m.nhid) ||
+      q->dH1.need((size_t)nb * m.nhid) || q->dX0.need((size_t)nb * m.nx()))
+    return -1;
+  q->nbmax = nb;
+  return 0;
+}
+
+static void mlp_gemm(hipStream_t st, int M, int N, int K, const float* A,
+                     long ai, long ak, const float* B, long bj, long bk,
+                     float* C, long ldc, const float* bias, int act, int acc) {
+  dim3 grid((N + MLP_T - 1) / MLP_T, (M + MLP_T - 1) / MLP_T);
+  hipLaunchKernelGGL(k_mlp_gemm, grid, dim3(MLP_T * MLP_T), 0, st, M, N, K,
+                     A, ai, ak, B, bj, bk, C, ldc, bias, act, acc);
+}
+
+// Forward pass of nb rows (X0 already gathered) on stream st:
+// Z1 = tanh(X0 W1^T + b1), Z2 = tanh(Z1 W2^T + b2), Y = x + Z2 W3^T + b3
+// (Z1, Z2 hold the activations).
+static void mlp_forward_rows(mlcp_mlp* q, int nb, hipStream_t st) {
+  const MlpDims& m = q->m;
+  const float* P = q->P.p;
+  mlp_gemm(st, nb, m.nhid, m.nx(), q->X0.p, m.nx(), 1, P + m.oW1(), m.nx(),
+           1, q->Z1.p, m.nhid, P + m.ob1(), 1, 0);
+  mlp_gemm(st, nb, m.nhid, m.nhid, q->Z1.p, m.nhid, 1, P + m.oW2(),
+           m.nhid, 1, q->Z2.p, m.nhid, P + m.ob2(), 1, 0);
+  hipLaunchKernelGGL(k_mlp_skip, dim3(256), dim3(256), 0, st, nb, m.nout,
+                     m.nx(), (const float*)q->X0.p, q->Y.p);
+  mlp_gemm(st, nb, m.nout, m.nhid, q->Z2.p, m.nhid, 1, P + m.oW3(),
+           m.nhid, 1, q->Y.p, m.nout, P + m.ob3(), 0, 1);
+}
+
+// Loss and gradient (into G) of nb rows idx[] of the set (X, modes, T).
+static int mlp_loss_grad_rows(mlcp_mlp* q, const float* X, const int* modes,
+                              const float* T, const int* idx, int nb) {
+  const MlpDims& m = q->m;
+  if (mlp_batch_bufs(q, nb) || q->sse.need(1)) return -1;
+  hipLaunchKernelGGL(k_mlp_input, dim3(256), dim3(256), 0, q->st, m, X, modes,
+                     idx, nb, q->P.p, q->X0.p);
+  mlp_forward_rows(q, nb, q->st);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, q->st, q->sse.p, 1L, 0.0);
+  hipLaunchKernelGGL(k_mlp_dy, dim3(256), dim3(256), 0, q->st, nb, m.nout,
+                     q->Y.p, T, idx, 1.0f / nb, q->dY.p, q->sse.p);
+  float* G = q->G.p;
+  // layer 3: dW3 = dY^T Z2, db3 = sum dY, dH2 = dY W3
+  mlp_gemm(q->st, m.nout, m.nhid, nb, q->dY.p, 1, m.nout, q->Z2.p, 1, m.nhid,
+           G + m.oW3(), m.nhid, nullptr, 0, 0);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3((m.nout + 63) / 64), dim3(64), 0,
+                     q->st, nb, m.nout, q->dY.p, G + m.ob3());
+  mlp_gemm(q->st, nb, m.nhid, m.nout, q->dY.p, m.nout, 1, q->P.p + m.oW3(), 1,
+           m.nhid, q->dH2.p, m.nhid, nullptr, 0, 0);
+  hipLaunchKernelGGL(k_mlp_dtanh, dim3(256), dim3(256), 0, q->st,
+                     (long)nb * m.nhid, q->Z2.p, q->dH2.p);
+  // layer 2
+  mlp_gemm(q->st, m.nhid, m.nhid, nb, q->dH2.p, 1, m.nhid, q->Z1.p, 1, m.nhid,
+           G + m.oW2(), m.nhid, nullptr, 0, 0);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3((m.nhid + 63) / 64), dim3(64), 0,
+                     q->st, nb, m.nhid, q->dH2.p, G + m.ob2());
+  mlp_gemm(q->st, nb, m.nhid, m.nhid, q->dH2.p, m.nhid, 1, q->P.p + m.oW2(),
+           1, m.nhid, q->dH1.p, m.nhid, nullptr, 0, 0);
+  hipLaunchKernelGGL(k_mlp_dtanh, dim3(256), dim3(256), 0, q->st,
+                     (long)nb * m.nhid, q->Z1.p, q->dH1.p);
+  // layer 1 and the embedding
+  mlp_gemm(q->st, m.nhid, m.nx(), nb, q->dH1.p, 1, m.nhid, q->X0.p, 1, m.nx(),
+           G + m.oW1(), m.nx(), nullptr, 0, 0);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3((m.nhid + 63) / 64), dim3(64), 0,
+                     q->st, nb, m.nhid, q->dH1.p, G + m.ob1());
+  mlp_gemm(q->st, nb, m.nx(), m.nhid, q->dH1.p, m.nhid, 1, q->P.p + m.oW1(),
+           1, m.nx(), q->dX0.p, m.nx(), nullptr, 0, 0);
+  hipLaunchKernelGGL(k_mlp_embgrad, dim3((m.nmodes * m.nemb + 63) / 64),
+                     dim3(64), 0, q->st, m, nb, modes, idx, q->dX0.p,
+                     G + m.oE());
+  HCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int mlcp_mlp_create(mlcp_mlp** out, int nin, int nemb, int nmodes,
+                               int nhid, unsigned long long seed) {
+  g_err.clear();
+  *out = nullptr;
+  if (nin < 1 || nin > MLCP_NMAX || nemb < 0 || nemb > 64 || nmodes < 1 ||
+      nmodes > MLCP_MAXD || nhid < 1 || nhid > 1024)
+    return fail("mlp_create: sizes out of range");
+  mlcp_mlp* q = new mlcp_mlp();
+  q->m = {nin, nemb, nmodes, nhid, nin};
+  const long np = q->m.np();
+  if (hipStreamCreateWithFlags(&q->st, hipStreamNonBlocking) != hipSuccess) {
+    delete q;
+    return fail("hipStreamCreate failed");
+  }
+  if (q->P.need(np) || q->G.need(np) || q->Mo.need(np) || q->Vo.need(np)) {
+    std::string e = g_err;
+    delete q;
+    g_err = e;
+    return -1;
+  }
+  // Glorot-uniform weights (splitmix64 stream from the seed), zero biases,
+  // embeddings uniform(-0.1, 0.1); Adam moments zero
+  std::vector<float> h(np, 0.0f);
+  unsigned long long z = seed;
+  auto uni = [&z]() {
+    z += 0x9E3779B97F4A7C15ull;
+    unsigned long long x = z;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const MlpDims& m = q->m;
+  auto fill = [&](long off, int rows, int cols) {
+    const double a = std::sqrt(6.0 / (rows + cols));
+    for (long e = 0; e < (long)rows * cols; e++)
+      h[off + e] = (float)((2.0 * uni() - 1.0) * a);
+  };
+  // the output layer starts at zero: the untrained network is the identity
+  // (the plain start)
+  fill(m.oW1(), m.nhid, m.nx());
+  fill(m.oW2(), m.nhid, m.nhid);
+  for (long e = 0; e < (long)m.nmodes * m.nemb; e++)
+    h[m.oE() + e] = (float)(0.2 * uni() - 0.1);
+  std::vector<float> zero(np, 0.0f);
+  if (hipMemcpy(q->P.p, h.data(), np * sizeof(float), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(q->Mo.p, zero.data(), np * sizeof(float),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(q->Vo.p, zero.data(), np * sizeof(float),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    delete q;
+    return fail("mlp_create: upload failed");
+  }
+  *out = q;
+  return 0;
+}
+
+extern "C" long mlcp_mlp_nparams(mlcp_mlp* q) { return q->m.np(); }
+
+extern "C" int mlcp_mlp_params(mlcp_mlp* q, float* host, int set) {
+  g_err.clear();
+  const size_t nb = sizeof(float) * q->m.np();
+  if (set)
+    HCHK(hipMemcpy(q->P.p, host, nb, hipMemcpyHostToDevice));
+  else
+    HCHK(hipMemcpy(host, q->P.p, nb, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int mlcp_mlp_destroy(mlcp_mlp* q) {
+  if (q) {
+    (void)hipStreamSynchronize(q->st);
+    delete q;
+  }
+  return 0;
+}
+
+// Uploads a set of N rows (X N x nin, modes N, T N x nout; T may be null).
+static int mlp_upload_set(mlcp_mlp* q, const float* X, const int* modes,
+                          const float* T, int N) {
+  const MlpDims& m = q->m;
+  for (int i = 0; i < N; i++)
+    if (modes[i] < 0 || modes[i] >= m.nmodes)
+      return fail("mlp: mode index out of range");
+  if (q->tX.need((size_t)N * m.nin) || q->tM.need(N) ||
+      (T && q->tT.need((size_t)N * m.nout)))
+    return -1;
+  HCHK(hipMemcpy(q->tX.p, X, sizeof(float) * N * m.nin, hipMemcpyHostToDevice));
+  HCHK(hipMemcpy(q->tM.p, modes, sizeof(int) * N, hipMemcpyHostToDevice));
+  if (T)
+    HCHK(hipMemcpy(q->tT.p, T, sizeof(float) * N * m.nout,
+                   hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int mlcp_mlp_loss_grad(mlcp_mlp* q, const float* X,
+                                  const int* modes, const float* T, int N,
+                                  double* loss, float* grad) {
+  g_err.clear();
+  if (N < 1) return fail("mlp: empty set");
+  if (mlp_upload_set(q, X, modes, T, N) ||
+      mlp_loss_grad_rows(q, q->tX.p, q->tM.p, q->tT.p, nullptr, N))
+    return -1;
+  double sse = 0.0;
+  HCHK(hipMemcpyAsync(&sse, q->sse.p, sizeof(double), hipMemcpyDeviceToHost,
+                      q->st));
+  HCHK(hipMemcpyAsync(grad, q->G.p, sizeof(float) * q->m.np(),
+                      hipMemcpyDeviceToHost, q->st));
+  HCHK(hipStreamSynchronize(q->st));
+  *loss = 0.5 * sse / N;
+  return 0;
+}
+
+extern "C" int mlcp_mlp_forward(mlcp_mlp* q, const float* X, const int* modes,
+                                int N, float* Y) {
+  g_err.clear();
+  if (N < 1) return fail("mlp: empty set");
+  if (mlp_upload_set(q, X, modes, nullptr, N) || mlp_batch_bufs(q, N))
+    return -1;
+  hipLaunchKernelGGL(k_mlp_input, dim3(256), dim3(256), 0, q->st, q->m,
+                     q->tX.p, q->tM.p, (const int*)nullptr, N, q->P.p,
+                     q->X0.p);
+  mlp_forward_rows(q, N, q->st);
+  HCHK(hipMemcpyAsync(Y, q->Y.p, sizeof(float) * N * q->m.nout,
+                      hipMemcpyDeviceToHost, q->st));
+  HCHK(hipStreamSynchronize(q->st));
+  return 0;
+}
+
+// Adam training on the set for `epochs` passes in shuffled minibatches of
+// `batch` rows (the order from a splitmix64 stream of `seed`); loss[e] =
+// mean loss of epoch e over its batches.
+extern "C" int mlcp_mlp_train(mlcp_mlp* q, const float* X, const int* modes,
+                              const float* T, int N, int epochs, int batch,
+                              float lr, unsigned long long seed,
+                              double* loss) {
+  g_err.clear();
+  if (N < 1 || epochs < 1 || batch < 1) return fail("mlp_train: bad sizes");
+  if (mlp_upload_set(q, X, modes, T, N) || q->tIdx.need(N) ||
+      mlp_batch_bufs(q, std::min(batch, N)))
+    return -1;
+  std::vector<int> perm(N);
+  unsigned long long z = seed;
+  auto next = [&z]() {
+    z += 0x9E3779B97F4A7C15ull;
+    unsigned long long x = z;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+  };
+  const long np = q->m.np();
+  for (int ep = 0; ep < epochs; ep++) {
+    for (int i = 0; i < N; i++) perm[i] = i;
+    for (int i = N - 1; i > 0; i--) std::swap(perm[i], perm[next() % (i + 1)]);
+    HCHK(hipMemcpyAsync(q->tIdx.p, perm.data(), sizeof(int) * N,
+                        hipMemcpyHostToDevice, q->st));
+    double tot = 0.0;
+    int nbat = 0;
+    for (int b0 = 0; b0 < N; b0 += batch) {
+      const int nb = std::min(batch, N - b0);
+      if (mlp_loss_grad_rows(q, q->tX.p, q->tM.p, q->tT.p, q->tIdx.p + b0, nb))
+        return -1;
+      q->step++;
+      hipLaunchKernelGGL(k_mlp_adam, dim3(256), dim3(256), 0, q->st, np,
+                         q->P.p, (const float*)q->G.p, q->Mo.p, q->Vo.p, lr,
+                         0.9f, 0.999f, 1e-8f, q->step);
+      double sse = 0.0;
+      HCHK(hipMemcpyAsync(&sse, q->sse.p, sizeof(double),
+                          hipMemcpyDeviceToHost, q->st));
+      HCHK(hipStreamSynchronize(q->st));
+      tot += 0.5 * sse / nb;
+      nbat++;
+    }
+    if (loss) loss[ep] = tot / nbat;
+  }
+  HCHK(hipGetLastError());
+  return 0;
+}
+
+// Warm start of the work columns c0 .. c0+ncol: every factor column on
+// every mode replaced by the network's normalised prediction.  Runs on the
+// session's stream (after the copy of the block into the work region).
+static int mlp_warm_start(mlcp_session* s, int c0, int ncol) {
+  mlcp_mlp* q = s->ws_mlp;
+  const MlpDims& m = q->m;
+  const int rows = ncol * s->ndof;
+  if (q->wX.need((size_t)rows * m.nin) || q->wM.need(rows) ||
+      mlp_batch_bufs(q, rows))
+    return -1;
+  // the network's work buffers belong to the MLP's stream: order the two
+  HCHK(hipStreamSynchronize(q->st));
+  hipLaunchKernelGGL(k_ws_gather, dim3(256), dim3(256), 0, s->st, s->pool, c0,
+                     ncol, m.nin, q->wX.p, q->wM.p);
+  hipLaunchKernelGGL(k_mlp_input, dim3(256), dim3(256), 0, s->st, m, q->wX.p,
+                     q->wM.p, (const int*)nullptr, rows, q->P.p, q->X0.p);
+  mlp_forward_rows(q, rows, s->st);
+  hipLaunchKernelGGL(k_ws_scatter, dim3(256), dim3(256), 0, s->st, s->pool,
+                     c0, ncol, m.nout, (const float*)q->Y.p);
+  HCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_set_warmstart(mlcp_session* s, mlcp_mlp* q) {
+  g_err.clear();
+  if (q) {
+    if (q->m.nin < s->nmax || q->m.nmodes < s->ndof)
+      return fail("set_warmstart: network smaller than the session's modes");
+  }
+  s->ws_mlp = q;
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// Device-resident solve: the cycle loop of SolveHPsi (Solver.f90:389-442)
+// with GetBlockShift on the device, each cycle one HIP graph.
+
+// One cycle with the shift read from s->d_es (set by k_block_shift from the
+// eigenvalues in s->eig) and its record in the history; no host copies,
+// allocations or waits, so it can be captured.
+static int solve_cycle_body(mlcp_session* s, int npow, int nals, double lo,
+                            double hi, int maxc) {
+  hipLaunchKernelGGL(k_block_shift, dim3(1), dim3(64), 0, s->st,
+                     (const double*)s->eig.p, s->nbloc, lo, hi, s->d_es.p);
+  s->es_dev = s->d_es.p;
+  int rc = phase_power(s, npow, 1, 0.0);
+  s->es_dev = nullptr;
+  if (rc || phase_ortho(s, nals) || phase_prodhv(s, nals) || phase_gep(s) ||
+      phase_update(s, s->U.p, nals))
+    return -1;
+  hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, s->st,
+                     (const double*)s->eig.p, (const double*)s->d_es.p,
+                     s->nbloc, s->d_hist.p, s->d_cnt.p, maxc);
+  HCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_solve(mlcp_session* s, double* eigv,
+                                      const double* bounds, int ncycle,
+                                      int npow, int nals, double* hist) {
+  g_err.clear();
+  const int B = s->nbloc;
+  if (ncycle < 1 || npow < 1 || nals < 1)
+    return fail("solve: ncycle, npow and nals must be positive");
+  if (s->form & FORM_ML)
+    return fail("solve: the learned warm start runs host-side steps; use "
+                "mlcp_hip_session_cycle for that form");
+  if (s->d_es.need(1) || s->d_cnt.need(1) ||
+      s->d_hist.need((size_t)ncycle * (B + 1)))
+    return -1;
+  HCHK(hipMemcpyAsync(s->eig.p, eigv, sizeof(double) * B,
+                      hipMemcpyHostToDevice, s->st));
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s->st, s->err.p, 1L, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s->st, s->d_cnt.p, 1L,
+                     0);
+  const double lo = bounds[0], hi = bounds[1];
+  // cycle 1 eagerly: every buffer the cycle needs is allocated and every
+  // table uploaded before a cycle is captured
+  if (solve_cycle_body(s, npow, nals, lo, hi, ncycle)) return -1;
+  const bool graph = use_graphs() && s->probe_kid == 0 && ncycle > 1;
+  if (graph) {
+    char key[160];
+    snprintf(key, sizeof key, "%d/%d/%.17g/%.17g/%d/%d/%.17g/%.17g", npow,
+             nals, lo, hi, ncycle, s->form, s->gs_tol, s->als_tol);
+    if (!s->cyc_exec || s->cyc_key != key) {
+      if (s->cyc_exec) {
+        HCHK(hipGraphExecDestroy(s->cyc_exec));
+        s->cyc_exec = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      HCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+      s->capturing = true;
+      int rc = solve_cycle_body(s, npow, nals, lo, hi, ncycle);
+      s->capturing = false;
+      hipError_t e = hipStreamEndCapture(s->st, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return -1;
+      }
+      if (e != hipSuccess)
+        return fail(std::string("cycle capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&s->cyc_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess)
+        return fail(std::string("cycle instantiate: ") + hipGetErrorString(e));
+      s->cyc_key = key;
+    }
+    for (int c = 1; c < ncycle; c++) HCHK(hipGraphLaunch(s->cyc_exec, s->st));
+  } else {
+    for (int c = 1; c < ncycle; c++)
+      if (solve_cycle_body(s, npow, nals, lo, hi, ncycle)) return -1;
+  }
+  if (check_err(s, "solve") || probe_collect(s)) return -1;
+  std::vector<double> h((size_t)ncycle * (B + 1));
+  HCHK(hipMemcpyAsync(h.data(), s->d_hist.p, sizeof(double) * h.size(),
+                      hipMemcpyDeviceToHost, s->st));
+  HCHK(hipMemcpyAsync(eigv, s->eig.p, sizeof(double) * B,
+                      hipMemcpyDeviceToHost, s->st));
+  HCHK(hipStreamSynchronize(s->st));
+  if (hist) memcpy(hist, h.data(), sizeof(double) * h.size());
   return 0;
 }
 
@@ -1116,11 +1618,19 @@ static int tune_set(mlcp_session* s, int had_tile, int had_split, int gs_xst) {
   if (had_tile && had_tile != s->had_tile) { s->had_tile = had_tile; jobs = true; }
   if (had_split && had_split != s->had_split) { s->had_split = had_split; jobs = true; }
   if (jobs && build_had_jobs(s)) return -1;
+  if (jobs && s->cyc_exec) {   // the captured cycle holds the old job list
+    HCHK(hipGraphExecDestroy(s->cyc_exec));
+    s->cyc_exec = nullptr;
+  }
   if (gs_xst && gs_xst != s->gs_xst) {
     s->gs_xst = gs_xst;
     if (s->gs_exec) {   // the captured Gram-Schmidt graph has the old grids
       HCHK(hipGraphExecDestroy(s->gs_exec));
       s->gs_exec = nullptr;
+    }
+    if (s->cyc_exec) {
+      HCHK(hipGraphExecDestroy(s->cyc_exec));
+      s->cyc_exec = nullptr;
     }
   }
   HCHK(hipStreamSynchronize(s->st));
@@ -1189,6 +1699,8 @@ extern "C" int mlcp_hip_session_tune_key(mlcp_session* s, char* buf, int len) {
 }
 
 extern "C" const char* mlcp_hip_last_error(void) { return g_err.c_str(); }
+
+extern "C" long mlcp_hip_device_bytes(void) { return (long)g_dev_bytes; }
 
 extern "C" int mlcp_hip_session_create(mlcp_session** out, const double* base,
                                        const double* coef, const int* nbas,
@@ -1282,6 +1794,35 @@ extern "C" int mlcp_hip_session_cycle(mlcp_session* s, double* eigv, int npow,
 
 extern "C" int mlcp_hip_session_phase_ms(mlcp_session* s, double* ms5) {
   for (int i = 0; i < 5; i++) ms5[i] = s->phase_ms[i];
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_set_form(mlcp_session* s, int form,
+                                         double gs_tol, double als_tol) {
+  g_err.clear();
+  if (form & ~FORM_ALL) return fail("set_form: unknown form bits");
+  if ((form & FORM_TOL) && !(gs_tol > 0.0 && als_tol > 0.0))
+    return fail("set_form: tolerances must be positive");
+  if (form != s->form && s->gs_exec) {   // captured with the old arguments
+    HCHK(hipGraphExecDestroy(s->gs_exec));
+    s->gs_exec = nullptr;
+  }
+  s->form = form;
+  s->gs_tol = gs_tol;
+  s->als_tol = als_tol;
+  return 0;
+}
+
+extern "C" int mlcp_hip_session_sweeps(mlcp_session* s, int* out3) {
+  g_err.clear();
+  const int B = s->nbloc;
+  HCHK(hipMemcpyAsync(out3, s->gsDone.p, sizeof(int) * B,
+                      hipMemcpyDeviceToHost, s->st));
+  HCHK(hipMemcpyAsync(out3 + B, s->opDone.p, sizeof(int) * B,
+                      hipMemcpyDeviceToHost, s->st));
+  HCHK(hipMemcpyAsync(out3 + 2 * B, s->lcDone.p, sizeof(int) * B,
+                      hipMemcpyDeviceToHost, s->st));
+  HCHK(hipStreamSynchronize(s->st));
   return 0;
 }
 
@@ -1742,6 +2283,10 @@ extern "C" void alscpp_(double* coef, double* base, int* nbas,
   dropin_end("alscpp_", alscpp_run(coef, base, nbas, ndof, rF, rG, nals));
 }
 
+// intwdevicemem_ (intertwining.cu:31-94) prints the device memory a call
+// needs.  The figures follow session_alloc and phase_update buffer by
+// buffer; the term list's slot count per mode is bounded by nops (the
+// argument list carries no term table), so the total is an upper bound.
 extern "C" void intwdevicemem_(const int* rk, const int* ndof,
                                const int* nbloc, const int* TxD,
                                const int* nops, const int* msize,
@@ -1749,28 +2294,43 @@ extern "C" void intwdevicemem_(const int* rk, const int* ndof,
                                const int* nbas) {
   (void)ncpu;
   (void)msize;
-  int D = *ndof, B = *nbloc, R = *rk, T = *TxD / std::max(1, D);
-  long nD = 0, nmax = 0;
-  for (int d = 0; d < D; d++) {
+  const double D = *ndof, B = *nbloc, R = *rk, T = *TxD / std::max(1, *ndof);
+  const double O = *nops;
+  double nD = 0, nmax = 0;
+  for (int d = 0; d < *ndof; d++) {
     nD += nbas[d];
-    nmax = std::max<long>(nmax, nbas[d]);
+    nmax = std::max<double>(nmax, nbas[d]);
   }
-  double MB = 1.0 / (1 << 20);
-  long N = (long)R * B;
-  double pool = (3.0 * N * (nD + 1)) * 8 * MB;
-  double terms = ((double)*nops * nmax * nmax * 8 + T * D * 4.0) * MB;
-  double opals = ((double)B * *nops * (R * R + nmax * R) +
-                  (double)B * D * R * R + 2.0 * B * R * R) * 8 * MB;
-  double gs = ((double)B * D * R * R + (B + 1.0) * R * nmax) * 8 * MB;
-  double upd = *update ? ((double)B * D * R * N + (double)B * N +
-                          (double)B * ((N + 63) / 64) * R * nmax) * 8 * MB
-                       : 0.0;
+  const double MB = 1.0 / (1 << 20), N = R * B, r2 = R * R;
+  // pool (3 regions), coefficients, per-state tables, penalties
+  const double pool = 8 * (3 * N * nD + 3 * N) + 4 * (3 * B + N) + 8 * B;
+  // dense operators (n_max^2 bound), term tables and Hadamard records
+  const double terms = 8 * O * nmax * nmax + 8 * T + 4 * T * D * 6 +
+                       8 * T * 3 + 16 * T * D;
+  // M, G, Y, W (slots <= nops), BB and the refresh/Hadamard job lists
+  const double opals = 8 * (B * O * r2 + B * D * r2 + B * O * nmax * R +
+                            B * O * r2 + B * r2) +
+                       8 * (2 * B * (O + D) + B * ((r2 + 63) / 64) * D * 4);
+  // Gram-Schmidt caches
+  const double gs = 8 * (B * D * r2 + D * r2 + (B + 1) * R * nmax + r2 +
+                         (B + 1) * GS_PCS + R * nmax + GJF_SIZE) + 4;
+  // pair jobs, QHQ and S, U, eigenvalues, eigensolver work
+  const double gep = 24.0 * B * (B + 1) + 8 * (2 * B * B + B * B + B +
+                                              4 * B * B + 2 * B + 8);
+  // update caches (allocated by the first cycle with update)
+  const double upd = *update
+      ? 8 * (B * D * R * N + B * D * r2 +
+             B * std::ceil(N / LC_CHUNK) * R * nmax + B * N) +
+            8 * B * std::ceil(N / LC_CHUNK)
+      : 0.0;
   printf(" *** MI355X device memory requirements *** \n \n");
-  printf(" Factor pool (3 regions)  : %10.2f MB \n", pool);
-  printf(" Term list (dense ops)    : %10.2f MB \n", terms);
-  printf(" Operator-ALS caches      : %10.2f MB \n", opals);
-  printf(" Gram-Schmidt caches      : %10.2f MB \n", gs);
-  if (*update) printf(" Update caches            : %10.2f MB \n", upd);
+  printf(" Factor pool (3 regions)  : %10.2f MB \n", pool * MB);
+  printf(" Term list (dense ops)    : %10.2f MB \n", terms * MB);
+  printf(" Operator-ALS caches      : %10.2f MB \n", opals * MB);
+  printf(" Gram-Schmidt caches      : %10.2f MB \n", gs * MB);
+  printf(" Block eigenproblem       : %10.2f MB \n", gep * MB);
+  if (*update) printf(" Update caches            : %10.2f MB \n", upd * MB);
   printf(" TOTAL                    : %10.2f MB \n \n",
-         pool + terms + opals + gs + upd);
+         (pool + terms + opals + gs + gep + upd) * MB);
+  fflush(stdout);
 }

@@ -160,6 +160,19 @@ class Session:
               "mlcp_hip_session_cycle")
         return ev
 
+    def solve(self, eigv, bounds, ncycle, npow, nals):
+        """ncycle cycles with the E-shift formed on the device
+        (mlcp_hip_session_solve): returns (final eigenvalues, per-cycle
+        eigenvalues (ncycle x nbloc), per-cycle shifts)."""
+        B = self.Q.nbloc
+        ev = np.ascontiguousarray(eigv, dtype=np.float64).copy()
+        hist = np.zeros(ncycle * (B + 1))
+        check(lib().mlcp_hip_session_solve(
+            self.h, ev, np.ascontiguousarray(bounds, dtype=np.float64),
+            ncycle, npow, nals, hist), "mlcp_hip_session_solve")
+        hist = hist.reshape(ncycle, B + 1)
+        return ev, hist[:, :B].copy(), hist[:, B].copy()
+
     def download(self) -> CPBlock:
         out = self.Q.copy()
         check(lib().mlcp_hip_session_download(self.h, out.base, out.coef),
@@ -226,6 +239,37 @@ class Session:
     def update(self, U, nals):
         u = np.ascontiguousarray(np.asarray(U, dtype=np.float64).T)
         check(lib().mlcp_hip_session_update(self.h, u, nals), "update")
+
+    FORM_TOL = 1
+    FORM_MIXED = 2
+    FORM_ML = 4
+
+    def set_form(self, form=0, gs_tol=1e-5, als_tol=1e-7):
+        """Form of the cycle (bits): 0 = the reference's fixed sweep counts
+        in FP64; FORM_TOL = Gram-Schmidt, the reduced H*Q and the updates
+        stop at a residual tolerance, with the fixed counts as the cap;
+        FORM_MIXED = the power sweeps apply the terms on the FP32 matrix
+        cores; FORM_ML = the reduced H*Q fit starts from the warm-start
+        network's factors (see set_warmstart)."""
+        check(lib().mlcp_hip_session_set_form(self.h, int(form),
+                                              float(gs_tol), float(als_tol)),
+              "set_form")
+
+    def set_warmstart(self, mlp):
+        """The network (mlcp_amd.warmstart.MLP) the FORM_ML form uses, or
+        None.  The session keeps a reference; the network must outlive its
+        use."""
+        self._ws = mlp
+        check(lib().mlcp_hip_session_set_warmstart(
+            self.h, mlp.h if mlp is not None else None), "set_warmstart")
+
+    def sweeps(self) -> np.ndarray:
+        """(3, nbloc): per state, the sweeps Gram-Schmidt, the reduced H*Q
+        and the updates needed in their last phase when they stopped at the
+        tolerance (0: the fixed count ran)."""
+        out = np.zeros(3 * self.Q.nbloc, dtype=np.int32)
+        check(lib().mlcp_hip_session_sweeps(self.h, out), "sweeps")
+        return out.reshape(3, self.Q.nbloc)
 
     def tune_set(self, had_tile=0, had_split=0, gs_stride=0):
         """Launch parameters (0 keeps a value); results do not depend on

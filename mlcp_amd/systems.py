@@ -122,6 +122,9 @@ class _Basis:
         self.nprim = nprim
         self.qk = ho_q_powers(nprim, 4)
         self.p2 = ho_p2(nprim)
+        # projected operators per (node, monomial restricted to the node):
+        # the many PES terms that share a factor on a group share its matrix
+        self._ops: Dict[Tuple[int, Monomial], np.ndarray] = {}
 
     def prim(self, mode: int, power: int) -> np.ndarray:
         return self.qk[power]
@@ -135,9 +138,15 @@ class _Basis:
                 if mm == m:
                     mat = self.prim(mm, k)
             return mat
-        mats = [self.op(c, mono) for c in node.children]
-        full = _kron_all(mats)
-        return node.vectors.T @ full @ node.vectors
+        inside = set(node.modes)
+        key = (id(node), tuple(x for x in mono if x[0] in inside))
+        got = self._ops.get(key)
+        if got is None:
+            mats = [self.op(c, mono) for c in node.children]
+            full = _kron_all(mats)
+            got = node.vectors.T @ full @ node.vectors
+            self._ops[key] = got
+        return got
 
     def leaf(self, mode: int) -> Layer:
         w = self.pes.omega[mode]
@@ -328,12 +337,19 @@ def hh111(nstates=40, rank=20, seed=20261203) -> Workload:
 
 
 def qff_workload(nmodes, nprim, group, keep, nstates, rank_lo, rank_hi, seed,
-                 name, npow=5, nals=10, ncycle=10) -> Workload:
+                 name, npow=5, nals=10, ncycle=10, tree=None,
+                 keeps=None) -> Workload:
+    """A synthetic force field solved up a mode tree: one level of groups of
+    ``group`` modes keeping ``keep`` eigenvectors, or the levels ``tree``
+    (group sizes per level, in children of the level below) keeping
+    ``keeps``; the top block of ``nstates`` states with ranks uniform in
+    rank_lo..rank_hi."""
     pes = qff_pes(nmodes, seed)
-    if nmodes % group:
-        raise ValueError("nmodes must be a multiple of the group size")
-    basis, groups = build_tree(pes, nprim, [[group] * (nmodes // group)],
-                               [keep])
+    if tree is None:
+        if nmodes % group:
+            raise ValueError("nmodes must be a multiple of the group size")
+        tree, keeps = [[group] * (nmodes // group)], [keep]
+    basis, groups = build_tree(pes, nprim, tree, keeps)
     H = top_layer_hoplist(basis, groups)
     rng = np.random.Generator(np.random.PCG64(seed + 1))
     ranks = rng.integers(rank_lo, rank_hi + 1, size=nstates)
@@ -354,3 +370,11 @@ def qff24(nstates=120, rank=32, seed=20261205) -> Workload:
     60, 120 states at psirank 32."""
     return qff_workload(24, 12, 3, 60, nstates, rank, rank, seed,
                         "qff24-d8n60")
+
+
+def qff48(nstates=200, seed=20261206) -> Workload:
+    """48-mode synthetic QFF, 10 HO functions per mode, a three-layer tree:
+    pairs keep 40, pairs of pairs (quads) keep 120, top block of 200 states
+    with ranks uniform in 16..64."""
+    return qff_workload(48, 10, 0, 0, nstates, 16, 64, seed, "qff48-d12n120",
+                        tree=[[2] * 24, [2] * 12], keeps=[40, 120])
