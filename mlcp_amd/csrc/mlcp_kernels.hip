@@ -893,32 +893,61 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_conv(OpArgs a, int sweep) {
 
 // Rayleigh quotients <F, H F> / <F, F> of every state from the operator
 // caches refreshed with Fold = F (RayleighQuotient2, BlockUtils.f90:658-688):
-// <F,HF> = sum_{i,l} c_i c_l sum_t coef_t prod_d M_{op(t,d)}(i,l).
-__global__ __launch_bounds__(MLCP_NT) void k_op_rq(OpArgs a, double* rq) {
+// <F,HF> = sum_{i,l} c_i c_l sum_t coef_t prod_d M_{op(t,d)}(i,l),
+// split over workgroups: workgroup (s, p) sums
+// <F, H F> and <F, F> over the elements e = p*RQ_E .. of state s's R x R
+// element grid (every term, every mode) into part[(s*np + p)*2 + {0, 1}];
+// k_op_rq_sum adds the np pieces of a state in order (so the result does
+// not depend on timing).  The single workgroup of k_op_rq per state left a
+// one-state session (the spectral-range probes) on one CU for 3.4 ms.
+#define RQ_E 64
+__global__ __launch_bounds__(MLCP_NT) void k_op_rq_part(OpArgs a, int np,
+                                                        double* part) {
   __shared__ double red[8];
-  int s = blockIdx.x, R = a.R[s], D = a.p.ndof;
+  const int s = blockIdx.x / np, p = blockIdx.x % np;
+  const int R = a.R[s], D = a.p.ndof;
   const double* c = a.p.coef + a.col[s];
   const double* Ms = a.M + (long)s * a.nops * a.rstr;
   const double* Gs = a.G + (long)s * D * a.rstr;
+  // 4 waves x 64 lanes: lane = element of this piece, wave = every 4th term
+  const int lane = TID & 63, w = TID >> 6;
+  const int e = p * RQ_E + lane;
   double num = 0.0, den = 0.0;
-  for (int e = TID; e < R * R; e += MLCP_NT) {
-    int i = e % R, l = e / R;
-    double cc = c[i] * c[l];
+  if (e < R * R) {
+    const int i = e % R, l = e / R;
+    const double cc = c[i] * c[l];
     double h = 0.0;
-    for (int t = 0; t < a.T; t++) {
+    for (int t = w; t < a.T; t += MLCP_NT >> 6) {
       const int* to = a.top + (long)t * D;
-      double p = a.tcoef[t];
-      for (int d = 0; d < D; d++) p *= Ms[(long)to[d] * a.rstr + e];
-      h += p;
+      double q = a.tcoef[t];
+      for (int d = 0; d < D; d++) q *= Ms[(long)to[d] * a.rstr + e];
+      h += q;
     }
-    double g = 1.0;
-    for (int d = 0; d < D; d++) g *= Gs[(long)d * a.rstr + e];
-    num += cc * h;
-    den += cc * g;
+    num = cc * h;
+    if (w == 0) {
+      double g = 1.0;
+      for (int d = 0; d < D; d++) g *= Gs[(long)d * a.rstr + e];
+      den = cc * g;
+    }
   }
   num = wg_sum(num, red);
   den = wg_sum(den, red);
-  if (TID == 0) rq[s] = num / den;
+  if (TID == 0) {
+    part[2 * (long)blockIdx.x] = num;
+    part[2 * (long)blockIdx.x + 1] = den;
+  }
+}
+
+__global__ void k_op_rq_sum(int np, const double* part, double* rq) {
+  const int s = blockIdx.x;
+  if (threadIdx.x == 0) {
+    double num = 0.0, den = 0.0;
+    for (int p = 0; p < np; p++) {
+      num += part[2 * ((long)s * np + p)];
+      den += part[2 * ((long)s * np + p) + 1];
+    }
+    rq[s] = num / den;
+  }
 }
 
 // GetBlockShift (BlockPower.f90:20-45) on the device: the shift of the next

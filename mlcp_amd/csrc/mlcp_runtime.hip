@@ -258,6 +258,7 @@ struct mlcp_session {
   const double* es_dev = nullptr;
   bool capturing = false;
   DBuf<double> d_es, d_hist;
+  DBuf<double> rqPart;   // Rayleigh-quotient pieces (k_op_rq_part)
   DBuf<int> d_cnt;
   hipGraphExec_t cyc_exec = nullptr;
   std::string cyc_key;
@@ -1901,8 +1902,13 @@ extern "C" int mlcp_hip_session_rayleigh(mlcp_session* s, double* rq) {
   g_err.clear();
   OpArgs a = op_args(s, false);
   if (refresh_all(s, a, 1)) return -1;
-  hipLaunchKernelGGL(k_op_rq, dim3(s->nbloc), dim3(MLCP_NT), 0, s->st, a,
-                     s->eig.p);
+  // element pieces of RQ_E per workgroup (the largest rank sets the count)
+  const int np = (s->Rmax * s->Rmax + RQ_E - 1) / RQ_E;
+  if (s->rqPart.need((size_t)2 * s->nbloc * np)) return -1;
+  hipLaunchKernelGGL(k_op_rq_part, dim3(s->nbloc * np), dim3(MLCP_NT), 0,
+                     s->st, a, np, s->rqPart.p);
+  hipLaunchKernelGGL(k_op_rq_sum, dim3(s->nbloc), dim3(64), 0, s->st, np,
+                     (const double*)s->rqPart.p, s->eig.p);
   HCHK(hipGetLastError());
   HCHK(hipMemcpyAsync(rq, s->eig.p, sizeof(double) * s->nbloc,
                       hipMemcpyDeviceToHost, s->st));

@@ -210,8 +210,14 @@ def build_tree(pes: PES, nprim: int, tree: Sequence[Sequence[int]],
                          kids)
             H = basis.hamiltonian(node)
             w, v = np.linalg.eigh(H)
+            v = v[:, :keep]
+            # canonical signs: the largest-magnitude entry of every kept
+            # eigenvector positive (LAPACK's signs depend on the BLAS
+            # kernels and thread count, and they enter every operator)
+            big = np.argmax(np.abs(v), axis=0)
+            v = v * np.where(v[big, np.arange(v.shape[1])] < 0, -1.0, 1.0)
             node.energies = w[:keep].copy()
-            node.vectors = v[:, :keep].copy()
+            node.vectors = v.copy()
             new.append(node)
         nodes = new
     return basis, nodes
