@@ -547,6 +547,48 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
 // has) and multiplies its sum by the identity product once.  Terms with more
 // than four other operators form groups of their own and take the plain
 // product over all modes.
+// A window of 64 consecutive records of the Hadamard term tables in a
+// wave's registers: lane j holds record base + j (operands and coefficient).
+struct RecWin {
+  int base;
+  int ox, oy, oz, ow;
+  int clo, chi;
+};
+
+__device__ __forceinline__ void recwin_load(RecWin& w, const int4* ops,
+                                            const double* coef, int base,
+                                            int re, int lane) {
+  // clamped to the last record of the range: every lane loads, always in
+  // bounds; entries past the range are never read
+  const int r = min(base + lane, re - 1);
+  const int4 o = ops[r];
+  const double c = coef[r];
+  w.base = base;
+  w.ox = o.x;
+  w.oy = o.y;
+  w.oz = o.z;
+  w.ow = o.w;
+  w.clo = __double2loint(c);
+  w.chi = __double2hiint(c);
+}
+
+// Makes record r (>= cur.base, records visited in increasing order) part of
+// the current window: when r leaves it, the prefetched window becomes the
+// current one and the window after it is requested.
+__device__ __forceinline__ void recwin_at(RecWin& cur, RecWin& nxt,
+                                          const int4* ops, const double* coef,
+                                          int r, int re, int lane) {
+  while (r - cur.base >= 64) {   // uniform
+    cur = nxt;
+    recwin_load(nxt, ops, coef, cur.base + 64, re, lane);
+  }
+}
+
+__device__ __forceinline__ double recwin_coef(const RecWin& w, int j) {
+  return __hiloint2double(__builtin_amdgcn_readlane(w.chi, j),
+                          __builtin_amdgcn_readlane(w.clo, j));
+}
+
 __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
                                                           const int2* jobs,
                                                           int k) {
@@ -581,8 +623,10 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
   }
   for (int i = TID; i < E; i += blockDim.x) Mc[a.nops * E + i] = 1.0;
   __syncthreads();
-  if (el >= nel) return;
-  if (part == 0) {
+  // lanes past the state's last element stay active to the end (the record
+  // windows are read across lanes by readlane); they store nothing
+  const bool live = el < nel;
+  if (part == 0 && live) {
     // BB = prod_{m=1..D-1} G_{(k+m)%D}, all factors loaded together
     const double* Gs = a.G + (long)s * D * a.rstr + e;
     double gv[MLCP_MAXD];
@@ -606,12 +650,22 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
   const cptr<unsigned> hg_mask = (cptr<unsigned>)(a.hg_mask);
   const cptr<int> hg_tb = (cptr<int>)(a.hg_tb);
   const cptr<int> hg_te = (cptr<int>)(a.hg_te);
-  const cptr<int> hr_ops = (cptr<int>)(a.hr_ops);   // 4 per record
-  const cptr<double> hr_coef = (cptr<double>)(a.hr_coef);
   const cptr<int> top = (cptr<int>)(a.top);
   const int h0 = hs_off[k];
   const int u0 = hsp[k * (a.nsplit + 1) + part];
   const int u1 = hsp[k * (a.nsplit + 1) + part + 1];
+  if (u0 >= u1) return;
+  // The records of this wave's slots are one contiguous range [rb, re).
+  // They stream through the wave's registers 64 at a time (lane j holds
+  // record base + j; the next window is loaded while the current one is
+  // used), and each record's operands reach the wave by readlane: a record
+  // costs its LDS reads and arithmetic, not a scalar-cache round trip (the
+  // tables of a mode exceed the scalar cache).
+  const int rb = hg_tb[hs_beg[u0]];
+  const int re = hg_te[hs_end[u1 - 1] - 1];
+  RecWin cur, nxt;
+  recwin_load(cur, a.hr_ops, a.hr_coef, rb, re, lane);
+  recwin_load(nxt, a.hr_ops, a.hr_coef, rb + 64, re, lane);
   for (int u = u0; u < u1; u++) {
     double acc = 0.0;
     for (int g = hs_beg[u]; g < hs_end[u]; g++) {
@@ -620,8 +674,10 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
       if (mask >> 31) {
         // terms with more than four other operators: record x = term id
         for (int r = tb; r < te; r++) {
-          const cptr<int> to = top + (long)hr_ops[4 * r] * D;
-          double pr = hr_coef[r];
+          recwin_at(cur, nxt, a.hr_ops, a.hr_coef, r, re, lane);
+          const int j = r - cur.base;
+          const cptr<int> to = top + (long)__builtin_amdgcn_readlane(cur.ox, j) * D;
+          double pr = recwin_coef(cur, j);
           for (int m = 1; m < D; m++) pr *= mc[to[(k + m) % D] * E];
           acc += pr;
         }
@@ -634,36 +690,109 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
       // (the modes other than k without the identity); the group's sum is
       // formed with exactly those and scaled by the identity product once
       const int nx = (D - 1) - __popc(mask);
-      double ag = 0.0;
-      if (nx >= 4) {
-#pragma unroll 8
-        for (int r = tb; r < te; r++) {
-          const cptr<int> o = hr_ops + 4 * r;
-          ag = fma(hr_coef[r], (mc[o[0] * E] * mc[o[1] * E]) *
-                                   (mc[o[2] * E] * mc[o[3] * E]), ag);
+      // four records at a time into four partial sums (their operand reads
+      // in flight together), then the group's sum
+      double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+      for (int r = tb; r < te;) {
+        recwin_at(cur, nxt, a.hr_ops, a.hr_coef, r, re, lane);
+        // the records of this window, no boundary inside the loops
+        const int rend = min(te, cur.base + 64);
+        if (nx >= 4) {
+          for (; r + 4 <= rend; r += 4) {
+            const int j = r - cur.base;
+            double t[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              const int o0 = __builtin_amdgcn_readlane(cur.ox, j + q);
+              const int o1 = __builtin_amdgcn_readlane(cur.oy, j + q);
+              const int o2 = __builtin_amdgcn_readlane(cur.oz, j + q);
+              const int o3 = __builtin_amdgcn_readlane(cur.ow, j + q);
+              t[q] = (mc[o0 * E] * mc[o1 * E]) * (mc[o2 * E] * mc[o3 * E]);
+            }
+            g0 = fma(recwin_coef(cur, j), t[0], g0);
+            g1 = fma(recwin_coef(cur, j + 1), t[1], g1);
+            g2 = fma(recwin_coef(cur, j + 2), t[2], g2);
+            g3 = fma(recwin_coef(cur, j + 3), t[3], g3);
+          }
+          for (; r < rend; r++) {
+            const int j = r - cur.base;
+            const int o0 = __builtin_amdgcn_readlane(cur.ox, j);
+            const int o1 = __builtin_amdgcn_readlane(cur.oy, j);
+            const int o2 = __builtin_amdgcn_readlane(cur.oz, j);
+            const int o3 = __builtin_amdgcn_readlane(cur.ow, j);
+            g0 = fma(recwin_coef(cur, j), (mc[o0 * E] * mc[o1 * E]) *
+                                              (mc[o2 * E] * mc[o3 * E]), g0);
+          }
+        } else if (nx == 3) {
+          for (; r + 4 <= rend; r += 4) {
+            const int j = r - cur.base;
+            double t[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              const int o0 = __builtin_amdgcn_readlane(cur.ox, j + q);
+              const int o1 = __builtin_amdgcn_readlane(cur.oy, j + q);
+              const int o2 = __builtin_amdgcn_readlane(cur.oz, j + q);
+              t[q] = (mc[o0 * E] * mc[o1 * E]) * mc[o2 * E];
+            }
+            g0 = fma(recwin_coef(cur, j), t[0], g0);
+            g1 = fma(recwin_coef(cur, j + 1), t[1], g1);
+            g2 = fma(recwin_coef(cur, j + 2), t[2], g2);
+            g3 = fma(recwin_coef(cur, j + 3), t[3], g3);
+          }
+          for (; r < rend; r++) {
+            const int j = r - cur.base;
+            const int o0 = __builtin_amdgcn_readlane(cur.ox, j);
+            const int o1 = __builtin_amdgcn_readlane(cur.oy, j);
+            const int o2 = __builtin_amdgcn_readlane(cur.oz, j);
+            g0 = fma(recwin_coef(cur, j), (mc[o0 * E] * mc[o1 * E]) * mc[o2 * E],
+                     g0);
+          }
+        } else if (nx == 2) {
+          for (; r + 4 <= rend; r += 4) {
+            const int j = r - cur.base;
+            double t[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              const int o0 = __builtin_amdgcn_readlane(cur.ox, j + q);
+              const int o1 = __builtin_amdgcn_readlane(cur.oy, j + q);
+              t[q] = mc[o0 * E] * mc[o1 * E];
+            }
+            g0 = fma(recwin_coef(cur, j), t[0], g0);
+            g1 = fma(recwin_coef(cur, j + 1), t[1], g1);
+            g2 = fma(recwin_coef(cur, j + 2), t[2], g2);
+            g3 = fma(recwin_coef(cur, j + 3), t[3], g3);
+          }
+          for (; r < rend; r++) {
+            const int j = r - cur.base;
+            const int o0 = __builtin_amdgcn_readlane(cur.ox, j);
+            const int o1 = __builtin_amdgcn_readlane(cur.oy, j);
+            g0 = fma(recwin_coef(cur, j), mc[o0 * E] * mc[o1 * E], g0);
+          }
+        } else if (nx == 1) {
+          for (; r + 4 <= rend; r += 4) {
+            const int j = r - cur.base;
+            double t[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+              t[q] = mc[__builtin_amdgcn_readlane(cur.ox, j + q) * E];
+            g0 = fma(recwin_coef(cur, j), t[0], g0);
+            g1 = fma(recwin_coef(cur, j + 1), t[1], g1);
+            g2 = fma(recwin_coef(cur, j + 2), t[2], g2);
+            g3 = fma(recwin_coef(cur, j + 3), t[3], g3);
+          }
+          for (; r < rend; r++) {
+            const int j = r - cur.base;
+            g0 = fma(recwin_coef(cur, j),
+                     mc[__builtin_amdgcn_readlane(cur.ox, j) * E], g0);
+          }
+        } else {
+          for (; r < rend; r++) g0 += recwin_coef(cur, r - cur.base);
         }
-      } else if (nx == 3) {
-#pragma unroll 8
-        for (int r = tb; r < te; r++) {
-          const cptr<int> o = hr_ops + 4 * r;
-          ag = fma(hr_coef[r], (mc[o[0] * E] * mc[o[1] * E]) * mc[o[2] * E], ag);
-        }
-      } else if (nx == 2) {
-#pragma unroll 8
-        for (int r = tb; r < te; r++) {
-          const cptr<int> o = hr_ops + 4 * r;
-          ag = fma(hr_coef[r], mc[o[0] * E] * mc[o[1] * E], ag);
-        }
-      } else if (nx == 1) {
-#pragma unroll 8
-        for (int r = tb; r < te; r++)
-          ag = fma(hr_coef[r], mc[hr_ops[4 * r] * E], ag);
-      } else {
-        for (int r = tb; r < te; r++) ag += hr_coef[r];
       }
+      const double ag = (g0 + g1) + (g2 + g3);
       acc = fma(ip, ag, acc);
     }
-    a.W[((long)s * a.maxslots + (u - h0)) * a.rstr + e] = acc;
+    if (live) a.W[((long)s * a.maxslots + (u - h0)) * a.rstr + e] = acc;
   }
 }
 
