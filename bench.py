@@ -43,7 +43,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 TOL = 0.01
-GS_TOL = 1e-5      # fast form: Gram-Schmidt residual tolerance
+GS_TOL = 3e-5      # fast form: Gram-Schmidt residual tolerance
 ALS_TOL = 1e-7     # fast form: fit-residual tolerance of the ALS fits
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0

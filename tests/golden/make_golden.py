@@ -213,7 +213,7 @@ def solve_oracle(w, bounds, ncycle, on_cycle=None):
 
 
 # cycles of the oracle's trajectory kept per workload (the 24-mode block
-# takes about 25 minutes per oracle cycle on 8 cores)
+# takes about 45 minutes per oracle cycle with 4 threads)
 GOLDEN_CYCLES = {"hh111": 10, "qff12": 10, "qff24": 2}
 
 

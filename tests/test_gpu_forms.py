@@ -9,9 +9,9 @@ counts), on the GPU through the C-ABI:
 Each is judged the way the north star states it: one cycle from the
 teacher-forced fixture block (tests/golden/teacher_<name>.npz) against the
 FP64 oracle's cycle from the same block, every eigenvalue within
-0.01 cm^-1.  Measured: tolerance form (1e-5 / 1e-7) 1.9e-4 (qff24) and
-3.9e-5 (qff12); mixed form 3.7e-6 and 1.6e-7 cm^-1 against the plain GPU
-form.
+0.01 cm^-1.  Measured against the oracle: tolerance form at Gram-Schmidt
+tolerance 1e-5 / 3e-5 (fits 1e-7) 1.1e-3 / 2.5e-3 cm^-1 (qff24) and
+2.6e-3 / 2.5e-3 (qff12).
 """
 import os
 import sys
@@ -54,15 +54,16 @@ def _cycle(w, z, Q, form=0, gs_tol=1e-5, als_tol=1e-7):
 def test_fast_forms_within_tolerance_of_the_oracle(hip, name):
     from mlcp_amd.intertwine import Session
     w, z, Q = _fixture(name)
-    for label, form in [("tolerance", Session.FORM_TOL),
-                        ("mixed", Session.FORM_MIXED),
-                        ("tolerance+mixed", Session.FORM_TOL |
-                         Session.FORM_MIXED)]:
-        eig, sw = _cycle(w, z, Q, form)
+    for label, form, gs_tol in [("tolerance", Session.FORM_TOL, 1e-5),
+                                ("tolerance", Session.FORM_TOL, 3e-5),
+                                ("mixed", Session.FORM_MIXED, 1e-5),
+                                ("tolerance+mixed", Session.FORM_TOL |
+                                 Session.FORM_MIXED, 3e-5)]:
+        eig, sw = _cycle(w, z, Q, form, gs_tol)
         err = np.abs(eig - z["eigv_out"])
         gs = np.where(sw[0] > 0, sw[0], w.nals)
-        say(name, label, "max |dE| vs oracle", f"{err.max():.3e}",
-            "mean Gram-Schmidt sweeps", f"{gs.mean():.2f}")
+        say(name, label, f"gs_tol {gs_tol:g}", "max |dE| vs oracle",
+            f"{err.max():.3e}", "mean Gram-Schmidt sweeps", f"{gs.mean():.2f}")
         assert np.all(err <= TF_TOL), (label, err.max())
         assert np.all(np.diff(eig) >= 0.0)
         if form & Session.FORM_TOL:
