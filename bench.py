@@ -233,6 +233,27 @@ def cpu_baseline(w, cores, converged_per_solve):
                                         for k in t}}
 
 
+def pmc_summary(path):
+    """Per kernel (template arguments dropped) and counter: dispatches, the
+    counter summed over them and its mean per dispatch, from a rocprofv3
+    counter_collection.csv (one row per dispatch and counter)."""
+    import csv
+    import re
+    acc = {}
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            name = re.sub(r"^void ", "", r["Kernel_Name"]).split("(")[0]
+            name = re.sub(r"<.*", "", name)
+            key = (name, r["Counter_Name"])
+            n, v = acc.get(key, (0, 0.0))
+            acc[key] = (n + 1, v + float(r["Counter_Value"]))
+    out = {}
+    for (name, ctr), (n, v) in sorted(acc.items()):
+        out.setdefault(name, {})[ctr] = {"dispatches": n, "sum": v,
+                                         "per_dispatch": v / n}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,10 +277,19 @@ def main():
                     help="counter runs: one Gram-Schmidt phase of the "
                          "workload's initial block (the launch mix of every "
                          "cycle), kernel by kernel, then exit")
+    ap.add_argument("--only-cycle", action="store_true",
+                    help="counter runs: one plain cycle of the workload's "
+                         "initial block, kernel by kernel, then exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="counter runs: one device-to-device copy of MIB MiB "
                          "(the counters' calibration), then exit")
+    ap.add_argument("--pmc-summary", default="", metavar="CSV",
+                    help="print per-kernel sums of a rocprofv3 "
+                         "counter_collection.csv as JSON, then exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(args.pmc_summary), indent=1))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must "
                           "be 1 (multi-GPU sharding is a design note only)"}))
@@ -290,6 +320,14 @@ def main():
     if args.ncycle:
         w.ncycle = args.ncycle
     golden, golden_status = load_golden(args.workload, w)
+
+    if args.only_cycle:
+        os.environ["MLCP_HIP_GRAPH"] = "0"
+        with Session(w.Q0, w.H) as s:
+            s.cycle(w.Q0.eigv, w.npow, w.nals, True, 1,
+                    block_shift(w.Q0.eigv, w.bounds))
+        print(json.dumps({"only_cycle": w.name}), flush=True)
+        return
 
     if args.only_gs:
         os.environ["MLCP_HIP_GRAPH"] = "0"
