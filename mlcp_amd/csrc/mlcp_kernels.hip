@@ -2039,8 +2039,8 @@ __global__ __launch_bounds__(MLCP_NT) void k_pair_inner(Pool p,
 // X = U^-1 V, sign: largest-|x| entry positive.  Work arrays in global memory
 // (ws: 4*n*n + 2*n doubles).  Output X column major n x n into Xo, w into wo.
 // MODE places the rotated matrix C and the accumulated rotations V: 0 both
-// in global memory, 1 C in LDS, 2 both in LDS (dynamic, n*n doubles each);
-// the host picks the largest that fits.
+// in global memory, 1 C in LDS, 2 both in LDS (dynamic: n*(n|1) doubles for
+// C, n*n for V); the host picks the largest that fits.
 #define GEP_U 4
 #define GEP_SWEEPS 60   // Jacobi sweep limit of k_gep
 template <int MODE>
@@ -2054,9 +2054,12 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
   __shared__ int pairs[2 * 128];
   __shared__ double rc[128], rs[128];
   int T = blockDim.x, tid = threadIdx.x;
+  // C in LDS has an odd leading dimension, so that the row pass (stride ldc
+  // between lanes) spreads over the banks
+  const int ldc = MODE >= 1 ? (n | 1) : n;
   double* U = ws;
   double* C = MODE >= 1 ? gsh : U + (long)n * n;
-  double* V = MODE == 2 ? gsh + (long)n * n : U + 2 * (long)n * n;
+  double* V = MODE == 2 ? gsh + (long)n * ldc : U + 2 * (long)n * n;
   double* Z = U + 3 * (long)n * n;
   double* wt = Z + (long)n * n;
   int* ord = (int*)(wt + n);
@@ -2064,7 +2067,7 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
   for (int e = tid; e < n * n; e += T) {
     int i = e % n, j = e / n;
     int a = min(i, j), b = max(i, j);
-    C[e] = Ain[a + (long)b * n];
+    C[i + (long)j * ldc] = Ain[a + (long)b * n];
     U[e] = Sin[a + (long)b * n];
   }
   if (tid == 0) sflag = 0;
@@ -2098,7 +2101,7 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
   // Z = C U^-1 (rows independent)
   for (int r = tid; r < n; r += T) {
     for (int j = 0; j < n; j++) {
-      double s = C[r + (long)j * n];
+      double s = C[r + (long)j * ldc];
       for (int k = 0; k < j; k++) s -= Z[r + (long)k * n] * U[k + (long)j * n];
       Z[r + (long)j * n] = s / U[j + (long)j * n];
     }
@@ -2108,17 +2111,17 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
   for (int c = tid; c < n; c += T) {
     for (int i = 0; i < n; i++) {
       double s = Z[i + (long)c * n];
-      for (int k = 0; k < i; k++) s -= U[k + (long)i * n] * C[k + (long)c * n];
-      C[i + (long)c * n] = s / U[i + (long)i * n];
+      for (int k = 0; k < i; k++) s -= U[k + (long)i * n] * C[k + (long)c * ldc];
+      C[i + (long)c * ldc] = s / U[i + (long)i * n];
     }
   }
   __syncthreads();
   for (int e = tid; e < n * n; e += T) {
     int i = e % n, j = e / n;
     if (i < j) {
-      double v = 0.5 * (C[i + (long)j * n] + C[j + (long)i * n]);
-      C[i + (long)j * n] = v;
-      C[j + (long)i * n] = v;
+      double v = 0.5 * (C[i + (long)j * ldc] + C[j + (long)i * ldc]);
+      C[i + (long)j * ldc] = v;
+      C[j + (long)i * ldc] = v;
     }
     V[e] = (i == j) ? 1.0 : 0.0;
   }
@@ -2135,9 +2138,11 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
   for (int sweep = 0; sweep <= GEP_SWEEPS; sweep++) {
     double off = 0.0, tot = 0.0;
     for (int e = tid; e < n * n; e += T) {
-      double v = C[e] * C[e];
+      const int i = e % n, j = e / n;
+      const double c = C[i + (long)j * ldc];
+      const double v = c * c;
       tot += v;
-      if (e % n != e / n) off += v;
+      if (i != j) off += v;
     }
     // block reduction of (off, tot)
     for (int o = 32; o > 0; o >>= 1) {
@@ -2174,9 +2179,9 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
         pairs[2 * q + 1] = p1;
         double c = 1.0, s = 0.0;
         if (p1 < n) {
-          double apq = C[p0 + (long)p1 * n];
+          double apq = C[p0 + (long)p1 * ldc];
           if (apq != 0.0) {
-            double app = C[p0 + (long)p0 * n], aqq = C[p1 + (long)p1 * n];
+            double app = C[p0 + (long)p0 * ldc], aqq = C[p1 + (long)p1 * ldc];
             double th = (aqq - app) / (2.0 * apq);
             double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
             c = 1.0 / sqrt(t * t + 1.0);
@@ -2193,7 +2198,7 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
       // flight together instead of one round trip per element.
       for (int e0 = 0; e0 < np * n; e0 += GEP_U * T) {
         double cx[GEP_U], cy[GEP_U], vx[GEP_U], vy[GEP_U], cc[GEP_U], ss[GEP_U];
-        long o0[GEP_U], o1[GEP_U];
+        long o0[GEP_U], o1[GEP_U], w0[GEP_U], w1[GEP_U];
         bool live[GEP_U];
 #pragma unroll
         for (int u = 0; u < GEP_U; u++) {
@@ -2202,22 +2207,24 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
           const int q = e < np * n ? qe : 0, r = e - qe * n;
           const int p0 = pairs[2 * q], p1 = pairs[2 * q + 1];
           live[u] = e < np * n && p1 < n && rs[q] != 0.0;
-          o0[u] = live[u] ? r + (long)p0 * n : 0;
-          o1[u] = live[u] ? r + (long)p1 * n : 0;
+          o0[u] = live[u] ? r + (long)p0 * ldc : 0;
+          o1[u] = live[u] ? r + (long)p1 * ldc : 0;
+          w0[u] = live[u] ? r + (long)p0 * n : 0;
+          w1[u] = live[u] ? r + (long)p1 * n : 0;
           cc[u] = rc[q];
           ss[u] = rs[q];
           cx[u] = C[o0[u]];
           cy[u] = C[o1[u]];
-          vx[u] = V[o0[u]];
-          vy[u] = V[o1[u]];
+          vx[u] = V[w0[u]];
+          vy[u] = V[w1[u]];
         }
 #pragma unroll
         for (int u = 0; u < GEP_U; u++)
           if (live[u]) {
             C[o0[u]] = cc[u] * cx[u] - ss[u] * cy[u];
             C[o1[u]] = ss[u] * cx[u] + cc[u] * cy[u];
-            V[o0[u]] = cc[u] * vx[u] - ss[u] * vy[u];
-            V[o1[u]] = ss[u] * vx[u] + cc[u] * vy[u];
+            V[w0[u]] = cc[u] * vx[u] - ss[u] * vy[u];
+            V[w1[u]] = ss[u] * vx[u] + cc[u] * vy[u];
           }
       }
       __syncthreads();
@@ -2234,8 +2241,8 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
           live[u] = e < np * n && p1 < n && rs[q] != 0.0;
           zr0[u] = r == p1;
           zr1[u] = r == p0;
-          o0[u] = live[u] ? p0 + (long)r * n : 0;
-          o1[u] = live[u] ? p1 + (long)r * n : 0;
+          o0[u] = live[u] ? p0 + (long)r * ldc : 0;
+          o1[u] = live[u] ? p1 + (long)r * ldc : 0;
           cc[u] = rc[q];
           ss[u] = rs[q];
           cx[u] = C[o0[u]];
@@ -2260,10 +2267,10 @@ __global__ __launch_bounds__(1024) void k_gep(const double* Ain,
   }
   // ascending order (rank sort, ties by index)
   for (int i = tid; i < n; i += T) {
-    double wi = C[i + (long)i * n];
+    double wi = C[i + (long)i * ldc];
     int r = 0;
     for (int j = 0; j < n; j++) {
-      double wj = C[j + (long)j * n];
+      double wj = C[j + (long)j * ldc];
       if (wj < wi || (wj == wi && j < i)) r++;
     }
     ord[r] = i;

@@ -985,9 +985,11 @@ static int launch_ortho(mlcp_session* s, int nals) {
 // LDS placement of the eigensolver's work matrices (k_gep MODE) for order n
 // and the dynamic LDS it needs.
 static int gep_mode(int n, size_t* lds) {
-  const size_t m = (size_t)n * n * sizeof(double);
-  if (2 * m <= lds_limit((const void*)k_gep<2>)) { *lds = 2 * m; return 2; }
-  if (m <= lds_limit((const void*)k_gep<1>)) { *lds = m; return 1; }
+  // C with the odd leading dimension n | 1, V with n
+  const size_t mc = (size_t)n * (n | 1) * sizeof(double);
+  const size_t mv = (size_t)n * n * sizeof(double);
+  if (mc + mv <= lds_limit((const void*)k_gep<2>)) { *lds = mc + mv; return 2; }
+  if (mc <= lds_limit((const void*)k_gep<1>)) { *lds = mc; return 1; }
   *lds = 0;
   return 0;
 }
