@@ -589,6 +589,34 @@ __device__ __forceinline__ double recwin_coef(const RecWin& w, int j) {
                           __builtin_amdgcn_readlane(w.clo, j));
 }
 
+// The same for the group tables: lane j holds group base + j (identity
+// mask, first and last record).
+struct GrpWin {
+  int base;
+  unsigned mask;
+  int tb, te;
+};
+
+__device__ __forceinline__ void grpwin_load(GrpWin& w, const unsigned* mask,
+                                            const int* tb, const int* te,
+                                            int base, int ge, int lane) {
+  const int g = min(base + lane, ge - 1);
+  w.base = base;
+  w.mask = mask[g];
+  w.tb = tb[g];
+  w.te = te[g];
+}
+
+__device__ __forceinline__ void grpwin_at(GrpWin& cur, GrpWin& nxt,
+                                          const unsigned* mask, const int* tb,
+                                          const int* te, int g, int ge,
+                                          int lane) {
+  while (g - cur.base >= 64) {   // uniform
+    cur = nxt;
+    grpwin_load(nxt, mask, tb, te, cur.base + 64, ge, lane);
+  }
+}
+
 __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
                                                           const int2* jobs,
                                                           int k) {
@@ -647,7 +675,6 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
   const cptr<int> hsp = (cptr<int>)(a.hsp);
   const cptr<int> hs_beg = (cptr<int>)(a.hs_beg);
   const cptr<int> hs_end = (cptr<int>)(a.hs_end);
-  const cptr<unsigned> hg_mask = (cptr<unsigned>)(a.hg_mask);
   const cptr<int> hg_tb = (cptr<int>)(a.hg_tb);
   const cptr<int> hg_te = (cptr<int>)(a.hg_te);
   const cptr<int> top = (cptr<int>)(a.top);
@@ -663,14 +690,29 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
   // tables of a mode exceed the scalar cache).
   const int rb = hg_tb[hs_beg[u0]];
   const int re = hg_te[hs_end[u1 - 1] - 1];
+  // the element's identity Grams, read once: a group's identity product is
+  // then formed in registers (modes in ascending order, as a product over
+  // the mask's modes)
+  double gid[MLCP_MAXD];
+#pragma unroll
+  for (int d = 0; d < MLCP_MAXD; d++)
+    gid[d] = (d < D && a.idop[d] >= 0) ? mc[a.idop[d] * E] : 1.0;
   RecWin cur, nxt;
   recwin_load(cur, a.hr_ops, a.hr_coef, rb, re, lane);
   recwin_load(nxt, a.hr_ops, a.hr_coef, rb + 64, re, lane);
+  // the groups of these slots, [gb, ge), stream the same way
+  const int gb = hs_beg[u0], ge = hs_end[u1 - 1];
+  GrpWin gcur, gnxt;
+  grpwin_load(gcur, a.hg_mask, a.hg_tb, a.hg_te, gb, ge, lane);
+  grpwin_load(gnxt, a.hg_mask, a.hg_tb, a.hg_te, gb + 64, ge, lane);
   for (int u = u0; u < u1; u++) {
     double acc = 0.0;
     for (int g = hs_beg[u]; g < hs_end[u]; g++) {
-      const unsigned mask = hg_mask[g];
-      const int tb = hg_tb[g], te = hg_te[g];
+      grpwin_at(gcur, gnxt, a.hg_mask, a.hg_tb, a.hg_te, g, ge, lane);
+      const int jg = g - gcur.base;
+      const unsigned mask = (unsigned)__builtin_amdgcn_readlane((int)gcur.mask, jg);
+      const int tb = __builtin_amdgcn_readlane(gcur.tb, jg);
+      const int te = __builtin_amdgcn_readlane(gcur.te, jg);
       if (mask >> 31) {
         // terms with more than four other operators: record x = term id
         for (int r = tb; r < te; r++) {
@@ -684,8 +726,9 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
         continue;
       }
       double ip = 1.0;
-      for (int d = 0; d < D; d++)
-        if ((mask >> d) & 1u) ip *= mc[a.idop[d] * E];
+#pragma unroll
+      for (int d = 0; d < MLCP_MAXD; d++)
+        if ((mask >> d) & 1u) ip *= gid[d];   // uniform: mask bits < D
       // every record of the group has the same number of explicit operators
       // (the modes other than k without the identity); the group's sum is
       // formed with exactly those and scaled by the identity product once
