@@ -617,6 +617,26 @@ __device__ __forceinline__ void grpwin_at(GrpWin& cur, GrpWin& nxt,
   }
 }
 
+// U records (window entries j .. j+U-1) of NX explicit operators each:
+// g + sum_q c_q prod_x M_{o_x(q)}, the products formed together, the sum in
+// record order.
+template <int NX, int U>
+__device__ __forceinline__ double had_run(const RecWin& w, int j,
+                                          const double* mc, int E, double g) {
+  double t[U];
+#pragma unroll
+  for (int q = 0; q < U; q++) {
+    double p = mc[__builtin_amdgcn_readlane(w.ox, j + q) * E];
+    if constexpr (NX >= 2) p *= mc[__builtin_amdgcn_readlane(w.oy, j + q) * E];
+    if constexpr (NX >= 3) p *= mc[__builtin_amdgcn_readlane(w.oz, j + q) * E];
+    if constexpr (NX >= 4) p *= mc[__builtin_amdgcn_readlane(w.ow, j + q) * E];
+    t[q] = p;
+  }
+#pragma unroll
+  for (int q = 0; q < U; q++) g = fma(recwin_coef(w, j + q), t[q], g);
+  return g;
+}
+
 __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
                                                           const int2* jobs,
                                                           int k) {
@@ -731,108 +751,37 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
         if ((mask >> d) & 1u) ip *= gid[d];   // uniform: mask bits < D
       // every record of the group has the same number of explicit operators
       // (the modes other than k without the identity); the group's sum is
-      // formed with exactly those and scaled by the identity product once
+      // formed with exactly those and scaled by the identity product once.
+      // The products of U records are formed together (their LDS reads in
+      // flight at once) and added in record order, so the sum does not
+      // depend on where the record windows start (the slot split)
       const int nx = (D - 1) - __popc(mask);
-      // four records at a time into four partial sums (their operand reads
-      // in flight together), then the group's sum
-      double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+      double ag = 0.0;
       for (int r = tb; r < te;) {
         recwin_at(cur, nxt, a.hr_ops, a.hr_coef, r, re, lane);
         // the records of this window, no boundary inside the loops
         const int rend = min(te, cur.base + 64);
-        if (nx >= 4) {
-          for (; r + 4 <= rend; r += 4) {
-            const int j = r - cur.base;
-            double t[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              const int o0 = __builtin_amdgcn_readlane(cur.ox, j + q);
-              const int o1 = __builtin_amdgcn_readlane(cur.oy, j + q);
-              const int o2 = __builtin_amdgcn_readlane(cur.oz, j + q);
-              const int o3 = __builtin_amdgcn_readlane(cur.ow, j + q);
-              t[q] = (mc[o0 * E] * mc[o1 * E]) * (mc[o2 * E] * mc[o3 * E]);
-            }
-            g0 = fma(recwin_coef(cur, j), t[0], g0);
-            g1 = fma(recwin_coef(cur, j + 1), t[1], g1);
-            g2 = fma(recwin_coef(cur, j + 2), t[2], g2);
-            g3 = fma(recwin_coef(cur, j + 3), t[3], g3);
-          }
-          for (; r < rend; r++) {
-            const int j = r - cur.base;
-            const int o0 = __builtin_amdgcn_readlane(cur.ox, j);
-            const int o1 = __builtin_amdgcn_readlane(cur.oy, j);
-            const int o2 = __builtin_amdgcn_readlane(cur.oz, j);
-            const int o3 = __builtin_amdgcn_readlane(cur.ow, j);
-            g0 = fma(recwin_coef(cur, j), (mc[o0 * E] * mc[o1 * E]) *
-                                              (mc[o2 * E] * mc[o3 * E]), g0);
-          }
-        } else if (nx == 3) {
-          for (; r + 4 <= rend; r += 4) {
-            const int j = r - cur.base;
-            double t[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              const int o0 = __builtin_amdgcn_readlane(cur.ox, j + q);
-              const int o1 = __builtin_amdgcn_readlane(cur.oy, j + q);
-              const int o2 = __builtin_amdgcn_readlane(cur.oz, j + q);
-              t[q] = (mc[o0 * E] * mc[o1 * E]) * mc[o2 * E];
-            }
-            g0 = fma(recwin_coef(cur, j), t[0], g0);
-            g1 = fma(recwin_coef(cur, j + 1), t[1], g1);
-            g2 = fma(recwin_coef(cur, j + 2), t[2], g2);
-            g3 = fma(recwin_coef(cur, j + 3), t[3], g3);
-          }
-          for (; r < rend; r++) {
-            const int j = r - cur.base;
-            const int o0 = __builtin_amdgcn_readlane(cur.ox, j);
-            const int o1 = __builtin_amdgcn_readlane(cur.oy, j);
-            const int o2 = __builtin_amdgcn_readlane(cur.oz, j);
-            g0 = fma(recwin_coef(cur, j), (mc[o0 * E] * mc[o1 * E]) * mc[o2 * E],
-                     g0);
-          }
-        } else if (nx == 2) {
-          for (; r + 4 <= rend; r += 4) {
-            const int j = r - cur.base;
-            double t[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              const int o0 = __builtin_amdgcn_readlane(cur.ox, j + q);
-              const int o1 = __builtin_amdgcn_readlane(cur.oy, j + q);
-              t[q] = mc[o0 * E] * mc[o1 * E];
-            }
-            g0 = fma(recwin_coef(cur, j), t[0], g0);
-            g1 = fma(recwin_coef(cur, j + 1), t[1], g1);
-            g2 = fma(recwin_coef(cur, j + 2), t[2], g2);
-            g3 = fma(recwin_coef(cur, j + 3), t[3], g3);
-          }
-          for (; r < rend; r++) {
-            const int j = r - cur.base;
-            const int o0 = __builtin_amdgcn_readlane(cur.ox, j);
-            const int o1 = __builtin_amdgcn_readlane(cur.oy, j);
-            g0 = fma(recwin_coef(cur, j), mc[o0 * E] * mc[o1 * E], g0);
-          }
-        } else if (nx == 1) {
-          for (; r + 4 <= rend; r += 4) {
-            const int j = r - cur.base;
-            double t[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-              t[q] = mc[__builtin_amdgcn_readlane(cur.ox, j + q) * E];
-            g0 = fma(recwin_coef(cur, j), t[0], g0);
-            g1 = fma(recwin_coef(cur, j + 1), t[1], g1);
-            g2 = fma(recwin_coef(cur, j + 2), t[2], g2);
-            g3 = fma(recwin_coef(cur, j + 3), t[3], g3);
-          }
-          for (; r < rend; r++) {
-            const int j = r - cur.base;
-            g0 = fma(recwin_coef(cur, j),
-                     mc[__builtin_amdgcn_readlane(cur.ox, j) * E], g0);
-          }
-        } else {
-          for (; r < rend; r++) g0 += recwin_coef(cur, r - cur.base);
+        switch (nx) {
+          case 1:
+            for (; r + 8 <= rend; r += 8) ag = had_run<1, 8>(cur, r - cur.base, mc, E, ag);
+            for (; r < rend; r++) ag = had_run<1, 1>(cur, r - cur.base, mc, E, ag);
+            break;
+          case 2:
+            for (; r + 8 <= rend; r += 8) ag = had_run<2, 8>(cur, r - cur.base, mc, E, ag);
+            for (; r < rend; r++) ag = had_run<2, 1>(cur, r - cur.base, mc, E, ag);
+            break;
+          case 3:
+            for (; r + 4 <= rend; r += 4) ag = had_run<3, 4>(cur, r - cur.base, mc, E, ag);
+            for (; r < rend; r++) ag = had_run<3, 1>(cur, r - cur.base, mc, E, ag);
+            break;
+          case 4:
+            for (; r + 4 <= rend; r += 4) ag = had_run<4, 4>(cur, r - cur.base, mc, E, ag);
+            for (; r < rend; r++) ag = had_run<4, 1>(cur, r - cur.base, mc, E, ag);
+            break;
+          default:   // no explicit operator: the coefficients alone
+            for (; r < rend; r++) ag += recwin_coef(cur, r - cur.base);
         }
       }
-      const double ag = (g0 + g1) + (g2 + g3);
       acc = fma(ip, ag, acc);
     }
     if (live) a.W[((long)s * a.maxslots + (u - h0)) * a.rstr + e] = acc;
