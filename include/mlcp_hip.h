@@ -186,7 +186,9 @@ int mlcp_hip_session_set_warmstart(mlcp_session *s, mlcp_mlp *q);
  * persisted instead of recomputed per call):
  *   had_tile   elements of R x R per Hadamard workgroup (multiple of 64, at
  *              most the LDS bound, reported as out4[3] by tune_get)
- *   had_split  slot ranges per Hadamard job (1..8; more, shorter jobs)
+ *   had_split  waves sharing each 64-element group of a Hadamard tile,
+ *              each over its own range of the mode's sub-slots (1, 2, 4, 8
+ *              or 16; had_tile/64 * had_split <= 16)
  *   gs_stride  Gram-Schmidt workgroup stride (1: all CUs, 8: one XCD)
  * 0 keeps a value.  Results are bitwise independent of all three.  A session
  * starts from the table $MLCP_HIP_TUNING (or tuning_table.txt beside the

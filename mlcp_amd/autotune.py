@@ -11,8 +11,8 @@ Knobs (``Session.tune_set``; none of them changes a result bit):
 * ``had_tile``  -- elements of the R x R Gram tile per cross-mode Hadamard
   workgroup (multiple of 64, at most what the operator Grams leave of LDS);
 * ``had_split`` -- waves sharing a 64-element group of that tile, each over
-  its own range of the mode's operator slots (1, 2 or 4; had_tile/64 x
-  had_split waves per workgroup, at most 4);
+  its own range of the mode's sub-slots (1, 2, 4, 8 or 16; had_tile/64 x
+  had_split waves per workgroup, at most 16);
 * ``gs_stride`` -- Gram-Schmidt workgroup stride (1: all CUs, 8: one XCD).
 
 Table format, one line per problem::
@@ -38,7 +38,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning_table.txt")
 
-SPLITS = (1, 2, 4)
+SPLITS = (1, 2, 4, 8, 16)
 STRIDES = (1, 8)
 
 
@@ -75,7 +75,7 @@ def write_entry(path: str, key: str, params: Tuple[int, int, int]) -> None:
 def candidates(had_tile_max: int) -> List[Tuple[int, int]]:
     """(had_tile, had_split) pairs of the Hadamard search."""
     tiles = [t for t in range(64, 257, 64) if t <= had_tile_max]
-    return [(t, p) for t in tiles for p in SPLITS if (t // 64) * p <= 4]
+    return [(t, p) for t in tiles for p in SPLITS if (t // 64) * p <= 16]
 
 
 def _measure(sess, eigv, npow, nals, eshift, reps):

@@ -17,6 +17,7 @@
 #define MLCP_NMAX 128     // basis functions per mode
 #define MLCP_BMAX 256     // states per block
 #define MLCP_NT 256       // threads per workgroup
+#define HAD_NT 1024       // most threads of a cross-mode Hadamard workgroup
 
 struct Pool {
   int ndof;
@@ -62,13 +63,21 @@ struct OpArgs {
   // once, each record its (<= 4) other operators
   int idop[MLCP_MAXD];   // identity operator of each mode (-1: none)
   const int* hs_off;     // [ndof+1] into hs_beg/hs_end (one entry per slot)
-  const int* hsp;        // [ndof][nsplit+1] slot ranges of the Hadamard jobs
-  int nsplit;            // slot ranges per (state, tile)
+  const int* hsp;        // [ndof][nsplit+1] sub-slot ranges of the parts
+  int nsplit;            // parts (waves) per 64-element group of a tile
   const int* hs_beg;     // [slot] first group
   const int* hs_end;
   const unsigned* hg_mask;  // [group] modes whose identity Gram is a factor
   const int* hg_tb;      // [group] first record
   const int* hg_te;
+  // sub-slots: a slot's groups cut into runs of about T/16 records (at
+  // group boundaries, fixed by the term list), the unit the parts share
+  const int* hss_off;    // [ndof+1] sub-slot range of each mode
+  const int* hss_gb;     // [sub-slot] first group
+  const int* hss_ge;     // [sub-slot] one past its last group
+  const int* hs_sb;      // [slot] first sub-slot
+  const int* hs_se;      // [slot] one past its last sub-slot
+  int nsubmax;           // most sub-slots of a mode (LDS rows of their sums)
   const double* hr_coef; // [record] term coefficient
   const int4* hr_ops;    // [record] other operators (-1: none); a term
                          // with more than four: x = term id, y = -2

@@ -637,13 +637,14 @@ __device__ __forceinline__ double had_run(const RecWin& w, int j,
   return g;
 }
 
-__global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
-                                                          const int2* jobs,
-                                                          int k) {
+__global__ __launch_bounds__(HAD_NT) void k_op_hadamard(OpArgs a,
+                                                         const int2* jobs,
+                                                         int k) {
   extern __shared__ double Mc[];
   // workgroup = G groups of 64 elements x P parts (one wave each): wave w
-  // takes elements of group w % G and the slots [hsp[k][p], hsp[k][p+1]) of
-  // part p = w / G; all waves of a group share its staged Grams
+  // takes elements of group w % G and the sub-slots [hsp[k][p],
+  // hsp[k][p+1]) of part p = w / G; all waves of a group share its staged
+  // Grams
   const int2 jb = jobs[blockIdx.x];
   if (a.done && a.done[jb.x]) return;   // converged state (tolerance form)
   const int P = a.nsplit, G = (blockDim.x >> 6) / P, E = 64 * G;
@@ -693,98 +694,118 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_hadamard(OpArgs a,
   // instead of a vector load (and its wait) per group of records.
   const cptr<int> hs_off = (cptr<int>)(a.hs_off);
   const cptr<int> hsp = (cptr<int>)(a.hsp);
-  const cptr<int> hs_beg = (cptr<int>)(a.hs_beg);
-  const cptr<int> hs_end = (cptr<int>)(a.hs_end);
   const cptr<int> hg_tb = (cptr<int>)(a.hg_tb);
   const cptr<int> hg_te = (cptr<int>)(a.hg_te);
   const cptr<int> top = (cptr<int>)(a.top);
   const int h0 = hs_off[k];
-  const int u0 = hsp[k * (a.nsplit + 1) + part];
-  const int u1 = hsp[k * (a.nsplit + 1) + part + 1];
-  if (u0 >= u1) return;
-  // The records of this wave's slots are one contiguous range [rb, re).
-  // They stream through the wave's registers 64 at a time (lane j holds
-  // record base + j; the next window is loaded while the current one is
-  // used), and each record's operands reach the wave by readlane: a record
-  // costs its LDS reads and arithmetic, not a scalar-cache round trip (the
-  // tables of a mode exceed the scalar cache).
-  const int rb = hg_tb[hs_beg[u0]];
-  const int re = hg_te[hs_end[u1 - 1] - 1];
-  // the element's identity Grams, read once: a group's identity product is
-  // then formed in registers (modes in ascending order, as a product over
-  // the mask's modes)
-  double gid[MLCP_MAXD];
+  const int h1 = hs_off[k + 1];
+  const cptr<int> hss_off = (cptr<int>)(a.hss_off);
+  const cptr<int> hss_gb = (cptr<int>)(a.hss_gb);
+  const cptr<int> hss_ge = (cptr<int>)(a.hss_ge);
+  const cptr<int> hs_sb = (cptr<int>)(a.hs_sb);
+  const cptr<int> hs_se = (cptr<int>)(a.hs_se);
+  const int sbase = hss_off[k];
+  const int sb0 = hsp[k * (a.nsplit + 1) + part];
+  const int sb1 = hsp[k * (a.nsplit + 1) + part + 1];
+  // sums of the sub-slots, one LDS row each (after the operator rows)
+  double* Ls = Mc + (long)(a.nops + 1) * E + el;
+  if (sb0 < sb1) {
+    // The records of this wave's sub-slots are one contiguous range
+    // [rb, re).  They stream through the wave's registers 64 at a time
+    // (lane j holds record base + j; the next window is loaded while the
+    // current one is used), and each record's operands reach the wave by
+    // readlane: a record costs its LDS reads and arithmetic, not a
+    // scalar-cache round trip (the tables of a mode exceed the scalar
+    // cache).
+    const int rb = hg_tb[hss_gb[sb0]];
+    const int re = hg_te[hss_ge[sb1 - 1] - 1];
+    // the element's identity Grams, read once: a group's identity product
+    // is then formed in registers (modes in ascending order, as a product
+    // over the mask's modes)
+    double gid[MLCP_MAXD];
 #pragma unroll
-  for (int d = 0; d < MLCP_MAXD; d++)
-    gid[d] = (d < D && a.idop[d] >= 0) ? mc[a.idop[d] * E] : 1.0;
-  RecWin cur, nxt;
-  recwin_load(cur, a.hr_ops, a.hr_coef, rb, re, lane);
-  recwin_load(nxt, a.hr_ops, a.hr_coef, rb + 64, re, lane);
-  // the groups of these slots, [gb, ge), stream the same way
-  const int gb = hs_beg[u0], ge = hs_end[u1 - 1];
-  GrpWin gcur, gnxt;
-  grpwin_load(gcur, a.hg_mask, a.hg_tb, a.hg_te, gb, ge, lane);
-  grpwin_load(gnxt, a.hg_mask, a.hg_tb, a.hg_te, gb + 64, ge, lane);
-  for (int u = u0; u < u1; u++) {
-    double acc = 0.0;
-    for (int g = hs_beg[u]; g < hs_end[u]; g++) {
-      grpwin_at(gcur, gnxt, a.hg_mask, a.hg_tb, a.hg_te, g, ge, lane);
-      const int jg = g - gcur.base;
-      const unsigned mask = (unsigned)__builtin_amdgcn_readlane((int)gcur.mask, jg);
-      const int tb = __builtin_amdgcn_readlane(gcur.tb, jg);
-      const int te = __builtin_amdgcn_readlane(gcur.te, jg);
-      if (mask >> 31) {
-        // terms with more than four other operators: record x = term id
-        for (int r = tb; r < te; r++) {
+    for (int d = 0; d < MLCP_MAXD; d++)
+      gid[d] = (d < D && a.idop[d] >= 0) ? mc[a.idop[d] * E] : 1.0;
+    RecWin cur, nxt;
+    recwin_load(cur, a.hr_ops, a.hr_coef, rb, re, lane);
+    recwin_load(nxt, a.hr_ops, a.hr_coef, rb + 64, re, lane);
+    // the groups of these sub-slots, [gb, ge), stream the same way
+    const int gb = hss_gb[sb0], ge = hss_ge[sb1 - 1];
+    GrpWin gcur, gnxt;
+    grpwin_load(gcur, a.hg_mask, a.hg_tb, a.hg_te, gb, ge, lane);
+    grpwin_load(gnxt, a.hg_mask, a.hg_tb, a.hg_te, gb + 64, ge, lane);
+    for (int sb = sb0; sb < sb1; sb++) {
+      double acc = 0.0;
+      for (int g = hss_gb[sb]; g < hss_ge[sb]; g++) {
+        grpwin_at(gcur, gnxt, a.hg_mask, a.hg_tb, a.hg_te, g, ge, lane);
+        const int jg = g - gcur.base;
+        const unsigned mask = (unsigned)__builtin_amdgcn_readlane((int)gcur.mask, jg);
+        const int tb = __builtin_amdgcn_readlane(gcur.tb, jg);
+        const int te = __builtin_amdgcn_readlane(gcur.te, jg);
+        if (mask >> 31) {
+          // terms with more than four other operators: record x = term id
+          for (int r = tb; r < te; r++) {
+            recwin_at(cur, nxt, a.hr_ops, a.hr_coef, r, re, lane);
+            const int j = r - cur.base;
+            const cptr<int> to = top + (long)__builtin_amdgcn_readlane(cur.ox, j) * D;
+            double pr = recwin_coef(cur, j);
+            for (int m = 1; m < D; m++) pr *= mc[to[(k + m) % D] * E];
+            acc += pr;
+          }
+          continue;
+        }
+        double ip = 1.0;
+#pragma unroll
+        for (int d = 0; d < MLCP_MAXD; d++)
+          if ((mask >> d) & 1u) ip *= gid[d];   // uniform: mask bits < D
+        // every record of the group has the same number of explicit operators
+        // (the modes other than k without the identity); the group's sum is
+        // formed with exactly those and scaled by the identity product once.
+        // The products of U records are formed together (their LDS reads in
+        // flight at once) and added in record order, so the sum does not
+        // depend on where the record windows start (the slot split)
+        const int nx = (D - 1) - __popc(mask);
+        double ag = 0.0;
+        for (int r = tb; r < te;) {
           recwin_at(cur, nxt, a.hr_ops, a.hr_coef, r, re, lane);
-          const int j = r - cur.base;
-          const cptr<int> to = top + (long)__builtin_amdgcn_readlane(cur.ox, j) * D;
-          double pr = recwin_coef(cur, j);
-          for (int m = 1; m < D; m++) pr *= mc[to[(k + m) % D] * E];
-          acc += pr;
+          // the records of this window, no boundary inside the loops
+          const int rend = min(te, cur.base + 64);
+          switch (nx) {
+            case 1:
+              for (; r + 8 <= rend; r += 8) ag = had_run<1, 8>(cur, r - cur.base, mc, E, ag);
+              for (; r < rend; r++) ag = had_run<1, 1>(cur, r - cur.base, mc, E, ag);
+              break;
+            case 2:
+              for (; r + 8 <= rend; r += 8) ag = had_run<2, 8>(cur, r - cur.base, mc, E, ag);
+              for (; r < rend; r++) ag = had_run<2, 1>(cur, r - cur.base, mc, E, ag);
+              break;
+            case 3:
+              for (; r + 4 <= rend; r += 4) ag = had_run<3, 4>(cur, r - cur.base, mc, E, ag);
+              for (; r < rend; r++) ag = had_run<3, 1>(cur, r - cur.base, mc, E, ag);
+              break;
+            case 4:
+              for (; r + 4 <= rend; r += 4) ag = had_run<4, 4>(cur, r - cur.base, mc, E, ag);
+              for (; r < rend; r++) ag = had_run<4, 1>(cur, r - cur.base, mc, E, ag);
+              break;
+            default:   // no explicit operator: the coefficients alone
+              for (; r < rend; r++) ag += recwin_coef(cur, r - cur.base);
+          }
         }
-        continue;
+        acc = fma(ip, ag, acc);
       }
-      double ip = 1.0;
-#pragma unroll
-      for (int d = 0; d < MLCP_MAXD; d++)
-        if ((mask >> d) & 1u) ip *= gid[d];   // uniform: mask bits < D
-      // every record of the group has the same number of explicit operators
-      // (the modes other than k without the identity); the group's sum is
-      // formed with exactly those and scaled by the identity product once.
-      // The products of U records are formed together (their LDS reads in
-      // flight at once) and added in record order, so the sum does not
-      // depend on where the record windows start (the slot split)
-      const int nx = (D - 1) - __popc(mask);
-      double ag = 0.0;
-      for (int r = tb; r < te;) {
-        recwin_at(cur, nxt, a.hr_ops, a.hr_coef, r, re, lane);
-        // the records of this window, no boundary inside the loops
-        const int rend = min(te, cur.base + 64);
-        switch (nx) {
-          case 1:
-            for (; r + 8 <= rend; r += 8) ag = had_run<1, 8>(cur, r - cur.base, mc, E, ag);
-            for (; r < rend; r++) ag = had_run<1, 1>(cur, r - cur.base, mc, E, ag);
-            break;
-          case 2:
-            for (; r + 8 <= rend; r += 8) ag = had_run<2, 8>(cur, r - cur.base, mc, E, ag);
-            for (; r < rend; r++) ag = had_run<2, 1>(cur, r - cur.base, mc, E, ag);
-            break;
-          case 3:
-            for (; r + 4 <= rend; r += 4) ag = had_run<3, 4>(cur, r - cur.base, mc, E, ag);
-            for (; r < rend; r++) ag = had_run<3, 1>(cur, r - cur.base, mc, E, ag);
-            break;
-          case 4:
-            for (; r + 4 <= rend; r += 4) ag = had_run<4, 4>(cur, r - cur.base, mc, E, ag);
-            for (; r < rend; r++) ag = had_run<4, 1>(cur, r - cur.base, mc, E, ag);
-            break;
-          default:   // no explicit operator: the coefficients alone
-            for (; r < rend; r++) ag += recwin_coef(cur, r - cur.base);
-        }
-      }
-      acc = fma(ip, ag, acc);
+      Ls[(long)(sb - sbase) * E] = acc;
     }
-    if (live) a.W[((long)s * a.maxslots + (u - h0)) * a.rstr + e] = acc;
+  }
+  // every sub-slot sum of the tile is in LDS: the waves of each element
+  // group share out the slots and add each slot's sub-slot sums in order
+  // (the same sum whatever the split)
+  __syncthreads();
+  if (live) {
+    for (int u = h0 + part; u < h1; u += P) {
+      double su = 0.0;
+      for (int sb = hs_sb[u]; sb < hs_se[u]; sb++) su += Ls[(long)(sb - sbase) * E];
+      a.W[((long)s * a.maxslots + (u - h0)) * a.rstr + e] = su;
+    }
   }
 }
 

@@ -201,10 +201,13 @@ struct mlcp_session {
   // grouped records of the cross-mode Hadamard
   int idop[MLCP_MAXD];
   std::vector<int> hs_off, hs_beg, hs_end, hg_tb, hg_te;
+  std::vector<int> hss_off, hss_gb, hss_ge, hs_sb, hs_se;
+  int nsubmax = 1;
   std::vector<unsigned> hg_mask;
   std::vector<double> hr_coef;
   std::vector<int4> hr_ops;
   DBuf<int> d_hs_off, d_hs_beg, d_hs_end, d_hg_tb, d_hg_te;
+  DBuf<int> d_hss_off, d_hss_gb, d_hss_ge, d_hs_sb, d_hs_se;
   DBuf<unsigned> d_hg_mask;
   DBuf<double> d_hr_coef;
   DBuf<int4> d_hr_ops;
@@ -372,10 +375,18 @@ static int build_terms(mlcp_session* s, const int* opid, const int* mptr,
   s->hg_mask.clear();
   s->hg_tb.clear();
   s->hg_te.clear();
+  s->hss_off.assign(D + 1, 0);
+  s->hss_gb.clear();
+  s->hss_ge.clear();
+  s->hs_sb.clear();
+  s->hs_se.clear();
+  // sub-slot size: about T/16 records (at least 32), whole groups
+  const long sub_len = std::max(32L, ((long)T + 15) / 16);
   s->hr_coef.clear();
   s->hr_ops.clear();
   for (int k = 0; k < D; k++) {
     s->hs_off[k] = (int)s->hs_beg.size();
+    s->hss_off[k] = (int)s->hss_gb.size();
     const int u0 = s->kslot_off[k], u1 = s->kslot_off[k + 1];
     const int* ord = s->korder.data() + s->kord_off[k];
     for (int u = u0; u < u1; u++) {
@@ -423,9 +434,25 @@ static int build_terms(mlcp_session* s, const int* opid, const int* mptr,
         s->hg_te.push_back((int)s->hr_coef.size());
       }
       s->hs_end.push_back((int)s->hg_mask.size());
+      // the slot's sub-slots: runs of whole groups of about sub_len records
+      s->hs_sb.push_back((int)s->hss_gb.size());
+      const int gs0 = s->hs_beg.back(), gs1 = s->hs_end.back();
+      for (int g = gs0; g < gs1;) {
+        const int gstart = g;
+        long n = 0;
+        while (g < gs1 && (g == gstart || n < sub_len)) {
+          n += s->hg_te[g] - s->hg_tb[g];
+          g++;
+        }
+        s->hss_gb.push_back(gstart);
+        s->hss_ge.push_back(g);
+      }
+      s->hs_se.push_back((int)s->hss_gb.size());
     }
+    s->nsubmax = std::max(s->nsubmax, (int)s->hss_gb.size() - s->hss_off[k]);
   }
   s->hs_off[D] = (int)s->hs_beg.size();
+  s->hss_off[D] = (int)s->hss_gb.size();
   std::vector<double> tc(hcoef, hcoef + T);
   hipStream_t st = s->st;
   if (s->opd.put(dense, st) || s->tcoef.put(tc, st) ||
@@ -439,6 +466,9 @@ static int build_terms(mlcp_session* s, const int* opid, const int* mptr,
       s->d_hs_off.put(s->hs_off, st) || s->d_hs_beg.put(s->hs_beg, st) ||
       s->d_hs_end.put(s->hs_end, st) || s->d_hg_mask.put(s->hg_mask, st) ||
       s->d_hg_tb.put(s->hg_tb, st) || s->d_hg_te.put(s->hg_te, st) ||
+      s->d_hss_off.put(s->hss_off, st) || s->d_hss_gb.put(s->hss_gb, st) ||
+      s->d_hss_ge.put(s->hss_ge, st) || s->d_hs_sb.put(s->hs_sb, st) ||
+      s->d_hs_se.put(s->hs_se, st) ||
       s->d_hr_coef.put(s->hr_coef, st) || s->d_hr_ops.put(s->hr_ops, st))
     return -1;
   return 0;
@@ -463,7 +493,7 @@ static int gs_stride() {
 // Largest Hadamard tile (multiple of 64, at most MLCP_NT) whose operator
 // Grams fit in LDS; 0 if not even 64 elements fit.
 static int had_tile_max(mlcp_session* s) {
-  const size_t per = (size_t)(s->nops + 1) * sizeof(double);
+  const size_t per = (size_t)(s->nops + 1 + s->nsubmax) * sizeof(double);
   const size_t lim = lds_limit((const void*)k_op_hadamard);
   for (int E = MLCP_NT; E >= 64; E -= 64)
     if (per * E <= lim) return E;
@@ -471,29 +501,31 @@ static int had_tile_max(mlcp_session* s) {
 }
 
 // Hadamard jobs (state, tile of had_tile elements; a workgroup runs
-// had_tile/64 x had_split waves) and, per mode, the slot ranges of the
-// parts: contiguous slots with about equal record counts.
+// had_tile/64 x had_split waves) and, per mode, the sub-slot ranges of the
+// parts: contiguous sub-slots with about equal record counts.
 static int build_had_jobs(mlcp_session* s) {
   const int D = s->ndof, P = s->had_split;
   std::vector<int> hsp((size_t)D * (P + 1));
+  auto recs = [&](int sub) {
+    long n = 0;
+    for (int g = s->hss_gb[sub]; g < s->hss_ge[sub]; g++)
+      n += s->hg_te[g] - s->hg_tb[g];
+    return n;
+  };
   for (int k = 0; k < D; k++) {
-    const int h0 = s->hs_off[k], h1 = s->hs_off[k + 1];
+    const int b0 = s->hss_off[k], b1 = s->hss_off[k + 1];
     long tot = 0;
-    for (int u = h0; u < h1; u++) {
-      for (int g = s->hs_beg[u]; g < s->hs_end[u]; g++)
-        tot += s->hg_te[g] - s->hg_tb[g];
-    }
-    int* b = hsp.data() + (size_t)k * (P + 1);
-    b[0] = h0;
+    for (int b = b0; b < b1; b++) tot += recs(b);
+    int* bp = hsp.data() + (size_t)k * (P + 1);
+    bp[0] = b0;
     long acc = 0;
     int p = 1;
-    for (int u = h0; u < h1 && p < P; u++) {
-      for (int g = s->hs_beg[u]; g < s->hs_end[u]; g++)
-        acc += s->hg_te[g] - s->hg_tb[g];
-      while (p < P && acc * P >= tot * p) b[p++] = u + 1;
+    for (int b = b0; b < b1 && p < P; b++) {
+      acc += recs(b);
+      while (p < P && acc * P >= tot * p) bp[p++] = b + 1;
     }
-    while (p < P) b[p++] = h1;
-    b[P] = h1;
+    while (p < P) bp[p++] = b1;
+    bp[P] = b1;
   }
   std::vector<int2> had;
   for (int i = 0; i < s->nbloc; i++) {
@@ -655,6 +687,12 @@ static OpArgs op_args(mlcp_session* s, bool work) {
   a.hg_mask = s->d_hg_mask.p;
   a.hg_tb = s->d_hg_tb.p;
   a.hg_te = s->d_hg_te.p;
+  a.hss_off = s->d_hss_off.p;
+  a.hss_gb = s->d_hss_gb.p;
+  a.hss_ge = s->d_hss_ge.p;
+  a.hs_sb = s->d_hs_sb.p;
+  a.hs_se = s->d_hs_se.p;
+  a.nsubmax = s->nsubmax;
   a.hr_coef = s->d_hr_coef.p;
   a.hr_ops = s->d_hr_ops.p;
   a.M = s->M.p;
@@ -824,7 +862,7 @@ static int op_als(mlcp_session* s, bool work, int nitn, int ishift,
     for (int k = 0; k < D; k++) {
       PLAUNCH(KID_OP_HADAMARD, k_op_hadamard, dim3(s->njobs_had),
               dim3(s->had_tile * s->had_split),
-              (size_t)(s->nops + 1) * s->had_tile * sizeof(double), a,
+              (size_t)(s->nops + 1 + s->nsubmax) * s->had_tile * sizeof(double), a,
               (const int2*)s->jobs_had.p, k);
       // row tiles of the largest rank of the block
       if (s->Rmax <= 16)
@@ -1609,13 +1647,14 @@ static int tune_set(mlcp_session* s, int had_tile, int had_split, int gs_xst) {
   const int tmax = had_tile_max(s);
   if (had_tile && (had_tile % 64 || had_tile < 64 || had_tile > tmax))
     return fail("had_tile must be a multiple of 64 in 64.." + std::to_string(tmax));
-  if (had_split && had_split != 1 && had_split != 2 && had_split != 4)
-    return fail("had_split must be 1, 2 or 4");
+  if (had_split && had_split != 1 && had_split != 2 && had_split != 4 &&
+      had_split != 8 && had_split != 16)
+    return fail("had_split must be 1, 2, 4, 8 or 16");
   const int t = had_tile ? had_tile : s->had_tile;
   const int p = had_split ? had_split : s->had_split;
-  if ((t / 64) * p * 64 > MLCP_NT)
+  if ((t / 64) * p * 64 > HAD_NT)
     return fail("had_tile / 64 * had_split must be at most " +
-                std::to_string(MLCP_NT / 64));
+                std::to_string(HAD_NT / 64));
   if (gs_xst && gs_xst != 1 && gs_xst != 8) return fail("gs_stride must be 1 or 8");
   bool jobs = false;
   if (had_tile && had_tile != s->had_tile) { s->had_tile = had_tile; jobs = true; }

@@ -27,10 +27,13 @@ def test_table_rejects_bad_lines(tmp_path):
 
 
 def test_candidates_respect_the_lds_bound():
-    assert autotune.candidates(256) == [(64, 1), (64, 2), (64, 4), (128, 1),
-                                        (128, 2), (192, 1), (256, 1)]
-    assert autotune.candidates(128) == [(64, 1), (64, 2), (64, 4), (128, 1),
-                                        (128, 2)]
+    assert autotune.candidates(256) == [
+        (64, 1), (64, 2), (64, 4), (64, 8), (64, 16), (128, 1), (128, 2),
+        (128, 4), (128, 8), (192, 1), (192, 2), (192, 4), (256, 1), (256, 2),
+        (256, 4)]
+    assert autotune.candidates(128) == [
+        (64, 1), (64, 2), (64, 4), (64, 8), (64, 16), (128, 1), (128, 2),
+        (128, 4), (128, 8)]
     assert autotune.candidates(0) == []
 
 
@@ -38,5 +41,5 @@ def test_committed_table_parses():
     t = autotune.read_table(autotune.TABLE)
     for k, (tile, split, stride) in t.items():
         assert k.count("/") == 6 and tile % 64 == 0 and 64 <= tile <= 256
-        assert split in (1, 2, 4) and stride in (1, 8)
-        assert tile // 64 * split <= 4
+        assert split in (1, 2, 4, 8, 16) and stride in (1, 8)
+        assert (tile // 64) * split <= 16

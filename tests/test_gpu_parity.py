@@ -385,7 +385,8 @@ def test_tuning_is_bitwise_neutral(hip, ncoup):
         d = s.tuning()
         e0 = s.cycle(Q.eigv, 3, 3, True, 1, 5.0)
         b0 = s.download()
-        for t, p, x in [(64, 1, 8), (128, 2, 1), (d["had_tile_max"], 1, 8)]:
+        for t, p, x in [(64, 1, 8), (128, 2, 1), (64, 8, 1), (64, 16, 1),
+                        (d["had_tile_max"], 1, 8)]:
             s.restore()
             s.tune_set(t, p, x)
             assert s.tuning()["had_tile"] == t
@@ -400,8 +401,8 @@ def test_tuning_rejects_bad_values(hip):
     say("tuning rejects")
     nb = [5, 6, 4]
     with Session(random_block(74, nb, [4, 4]), random_system(73, nb, ncoup=4)) as s:
-        for bad in [(100, 0, 0), (0, 3, 0), (0, 8, 0), (0, 0, 4), (1024, 0, 0),
-                    (256, 2, 0)]:
+        for bad in [(100, 0, 0), (0, 3, 0), (0, 32, 0), (0, 0, 4), (1024, 0, 0),
+                    (256, 8, 0)]:
             with pytest.raises(RuntimeError):
                 s.tune_set(*bad)
 
