@@ -24,7 +24,7 @@ Two forms run by default, each timed on its own solves:
          FP32 matrix cores, and the learned warm start of the reduced H*Q
          fit is trained (untimed) on pairs from plain cycles of the same
          Hamiltonian and used only if it beats the plain start on a held-out
-         block.
+         block (lower prediction error, half a sweep per state fewer).
 `value` is the fast form's; both forms' figures are in `forms`.
 
     python bench.py [--steps K] [--warmup W] [--workload qff24] [--form both]

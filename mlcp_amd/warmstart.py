@@ -13,7 +13,8 @@ predictions replace Psi's columns as the fit's starting point.
 
 The plain start stays the judge and the fallback: ``gain`` measures the
 sweeps both starts need to the same fit-residual tolerance on held-out
-blocks, and ``choose`` keeps the network only where it needs fewer.
+blocks, and ``choose`` keeps the network only where it needs at least
+half a sweep per state fewer.
 """
 from __future__ import annotations
 
@@ -158,10 +159,20 @@ def gain(session, mlp: MLP, nals: int, als_tol: float = 1e-8) -> Gain:
     return Gain(p, w, nals)
 
 
+# Reduced H*Q sweeps per state the network must save on the held-out block
+# to be kept.  A smaller gain is within the run-to-run spread (measured:
+# 6.53 against 6.55 and 6.32 against 6.31 sweeps on the 24-mode block), and
+# a start that ALS leaves earlier ends on a different truncated fit: on the
+# 24-mode block such a network moved the teacher-forced eigenvalues by up
+# to 0.1 cm^-1, against 2.5e-3 from the plain start.
+MIN_SWEEPS_SAVED = 0.5
+
+
 def choose(g: Gain) -> bool:
-    """Keep the network only if it needs fewer sweeps than the plain start
-    on the held-out block (the plain start is the fallback)."""
-    return g.mean_warm < g.mean_plain
+    """Keep the network only if it saves at least MIN_SWEEPS_SAVED sweeps
+    per state over the plain start on the held-out block (the plain start
+    is the fallback)."""
+    return g.mean_warm <= g.mean_plain - MIN_SWEEPS_SAVED
 
 
 @dataclass
@@ -184,8 +195,8 @@ def train_on_solve(session, w, bounds, ncycle: int, epochs: int = 20,
     every cycle from the ``skip``-th on, trains the network on all but the
     last cycle's pairs and judges it on the last: the held-out prediction
     error against the plain start's, and the reduced H*Q sweeps both starts
-    need to ``als_tol`` on the final block.  The network is kept only if it
-    is better on both counts."""
+    need to ``als_tol`` on the final block.  The network is kept only if its
+    held-out error is lower and it passes ``choose``."""
     from .intertwine import block_shift
     nin = int(np.max(session.Q.nbas))
     D = int(session.Q.nbas.size)
@@ -213,5 +224,5 @@ def train_on_solve(session, w, bounds, ncycle: int, epochs: int = 20,
     pred = float(np.mean(np.sum((Y - Ts[-1]) ** 2, axis=1)))
     ident = float(np.mean(np.sum((Xs[-1] - Ts[-1]) ** 2, axis=1)))
     g = gain(session, net, w.nals, als_tol)
-    used = pred < ident and g.mean_warm < g.mean_plain
+    used = pred < ident and choose(g)
     return Fit(net, int(Xtr.shape[0]), losses, pred, ident, g, used)

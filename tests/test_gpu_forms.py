@@ -97,7 +97,8 @@ def test_learned_warm_start_gain_and_fallback(hip):
     """The network trained on plain cycles of the 12-mode workload (its own
     forward, backward and Adam kernels), judged on a held-out block against
     the plain start: held-out prediction error and reduced H*Q sweeps to
-    the fit tolerance.  It is kept only if better on both; whichever start
+    the fit tolerance.  It is kept only if its error is lower and it saves
+    at least half a sweep per state; whichever start
     is chosen, the reduced H*Q it produces stays a fit of H Psi (finite,
     and the cycle's eigenvalues ascending)."""
     from mlcp_amd.intertwine import Session, block_shift
@@ -112,7 +113,7 @@ def test_learned_warm_start_gain_and_fallback(hip):
             fit.gain.mean_plain, "warm", fit.gain.mean_warm, "kept", fit.used)
         assert fit.losses[-1] < fit.losses[0]
         assert fit.used == (fit.pred_err < fit.ident_err and
-                            fit.gain.mean_warm < fit.gain.mean_plain)
+                            fit.gain.mean_warm <= fit.gain.mean_plain - 0.5)
         form = Session.FORM_TOL | (Session.FORM_ML if fit.used else 0)
         s.set_warmstart(fit.net if fit.used else None)
         s.set_form(form, 1e-5, 1e-7)
