@@ -231,7 +231,7 @@ int mlcp_hip_gep(int n, const double *A, const double *B, double *X,
  * counterpart in the reference.  dA, dB (upper triangles, column major
  * n x n), dX, dw and dinfo are device pointers owned by the caller; the work
  * matrices live in a fixed device array of the library, one per GPU, sized
- * for n <= 256 (about 2 MB); larger n is refused with an error.  The call is
+ * for n <= 256 (1 MB); larger n is refused with an error.  The call is
  * asynchronous on `stream` (a hipStream_t; 0 for the null stream) and only
  * enqueues kernels, so it can be captured.  *dinfo is 0 on success, else
  * error bits (4: B not positive definite, 8: non-finite or no convergence),

@@ -2047,265 +2047,469 @@ __global__ __launch_bounds__(MLCP_NT) void k_pair_inner(Pool p,
 
 // --------------------------------------------------------------------------
 // Generalized symmetric-definite eigenproblem A x = w S x (DSYGV ITYPE=1,
-// UPLO='U' semantics) in one workgroup.  Cholesky S = U^T U, C = U^-T A U^-1,
-// parallel (round-robin ordered) two-sided Jacobi, ascending sort,
-// X = U^-1 V, sign: largest-|x| entry positive.  Work arrays in global memory
-// (ws: 4*n*n + 2*n doubles).  Output X column major n x n into Xo, w into wo.
-// MODE places the rotated matrix C and the accumulated rotations V: 0 both
-// in global memory, 1 C in LDS, 2 both in LDS (dynamic: n*(n|1) doubles for
-// C, n*n for V); the host picks the largest that fits.
-#define GEP_U 4
-#define GEP_SWEEPS 60   // Jacobi sweep limit of k_gep
+// UPLO='U' semantics) in one workgroup of 1024 threads:
+//   1. Cholesky S = U^T U, right-looking (U to ws);
+//   2. C = U^-T A U^-1 by two right-looking triangular sweeps, symmetrised;
+//   3. Householder tridiagonalisation C = Q T Q^T (the vectors kept below
+//      the diagonal), then Q formed in place from them, last one first;
+//   4. implicit-shift QL on T, its plane rotations applied to the rows of Q
+//      (the tql2 algorithm: Bowdler, Martin, Reinsch and Wilkinson, Numer.
+//      Math. 11 (1968) 293-306).  Wave 0 forms one QL sweep's rotations
+//      while waves 1-4 apply the previous sweep's, one row of Q per thread;
+//   5. ascending sort, X = U^-1 Z (right-looking), sign: largest-|x| entry
+//      positive.
+// Every step of 1-3 and 5 touches O(n^2) entries between two barriers (O(n)
+// steps, O(n^3) work); 4 is a chain of about n^2 dependent rotations, two
+// sweeps per eigenvalue on average.  MODE 1
+// keeps the working matrix in LDS (dynamic: n*(n|1) doubles), MODE 0 in ws.
+// ws: U (n*n), then the working matrix (n*n, MODE 0).  Output X column major
+// n x n into Xo, w into wo.  Error bits (outputs untouched): 4 S not
+// positive definite, 8 non-finite entries or no QL convergence.
+#define GEP_QL_ITER 60   // QL iterations allowed per eigenvalue
+#define GEP_NT 1024
+// Block updates of m <= 256 rows: a pass covers 2^gep_sh(m) rows (64, 128 or
+// 256, the fewest that hold m) and GEP_NT >> gep_sh(m) column phases, so a
+// small trailing block still spreads over every thread.
+__device__ __forceinline__ int gep_sh(int m) { return m <= 64 ? 6 : (m <= 128 ? 7 : 8); }
 template <int MODE>
-__global__ __launch_bounds__(1024) void k_gep(const double* Ain,
-                                              const double* Sin, int n,
-                                              double* ws, double* Xo,
-                                              double* wo, int* err) {
+__global__ __launch_bounds__(GEP_NT) void k_gep(const double* Ain,
+                                                const double* Sin, int n,
+                                                double* ws, double* Xo,
+                                                double* wo, int* err) {
   extern __shared__ double gsh[];
-  __shared__ int sflag;
-  __shared__ double sred[32];
-  __shared__ int pairs[2 * 128];
-  __shared__ double rc[128], rs[128];
-  int T = blockDim.x, tid = threadIdx.x;
-  // C in LDS has an odd leading dimension, so that the row pass (stride ldc
-  // between lanes) spreads over the banks
-  const int ldc = MODE >= 1 ? (n | 1) : n;
+  __shared__ double s_u[2][MLCP_BMAX];     // staged row/column of U
+  __shared__ double s_v[2][MLCP_BMAX];     // Householder vector x of a step
+  __shared__ double s_p[MLCP_BMAX];        // p = b C v, then y = b v^T Q
+  __shared__ double s_d[MLCP_BMAX], s_e[MLCP_BMAX], s_b[MLCP_BMAX];
+  __shared__ double s_rc[2][MLCP_BMAX], s_rs[2][MLCP_BMAX];
+  __shared__ double s_part[2][GEP_NT / 64], s_q[GEP_NT / 64];
+  __shared__ int s_lohi[2][2];
+  __shared__ int s_flag[2];
+  __shared__ int s_ord[MLCP_BMAX];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int ld = MODE == 1 ? (n | 1) : n;
   double* U = ws;
-  double* C = MODE >= 1 ? gsh : U + (long)n * n;
-  double* V = MODE == 2 ? gsh + (long)n * ldc : U + 2 * (long)n * n;
-  double* Z = U + 3 * (long)n * n;
-  double* wt = Z + (long)n * n;
-  int* ord = (int*)(wt + n);
-  // symmetric copies of the upper triangles
-  for (int e = tid; e < n * n; e += T) {
-    int i = e % n, j = e / n;
-    int a = min(i, j), b = max(i, j);
-    C[i + (long)j * ldc] = Ain[a + (long)b * n];
-    U[e] = Sin[a + (long)b * n];
+  double* M = MODE == 1 ? gsh : ws + (long)n * n;
+#define GM(i, j) M[(i) + (long)(j) * ld]
+#define GU(i, j) U[(i) + (long)(j) * n]
+  // ---- 1. Cholesky: S (upper triangle) -> M, full symmetric
+  for (int e = tid; e < n * n; e += GEP_NT) {
+    const int i = e % n, j = e / n;
+    GM(i, j) = Sin[min(i, j) + (long)max(i, j) * n];
   }
-  if (tid == 0) sflag = 0;
   __syncthreads();
-  // Cholesky, upper: U(j,j) = sqrt(S(j,j) - sum_k U(k,j)^2); row j
   for (int j = 0; j < n; j++) {
-    if (tid == 0) {
-      double s = U[j + (long)j * n];
-      for (int k = 0; k < j; k++) s -= U[k + (long)j * n] * U[k + (long)j * n];
-      if (!(s > 0.0)) sflag = 1;
-      U[j + (long)j * n] = sqrt(fabs(s));
+    const double sjj = GM(j, j);   // the same value in every thread
+    if (!(sjj > 0.0)) {
+      if (tid == 0) atomicOr(err, 4);
+      return;
     }
+    const double ujj = sqrt(sjj);
+    for (int c = j + 1 + tid; c < n; c += GEP_NT) {
+      const double u = GM(j, c) / ujj;
+      GM(j, c) = u;
+      GU(j, c) = u;
+    }
+    if (tid == 0) GU(j, j) = ujj;
     __syncthreads();
-    double ujj = U[j + (long)j * n];
-    for (int c = j + 1 + tid; c < n; c += T) {
-      double s = U[j + (long)c * n];
-      for (int k = 0; k < j; k++) s -= U[k + (long)j * n] * U[k + (long)c * n];
-      U[j + (long)c * n] = s / ujj;
+    // trailing block M(i, c) -= U(j, i) U(j, c), i, c > j
+    const int mm = n - j - 1, sh = gep_sh(mm);
+    const int R = tid & ((1 << sh) - 1), CQ = tid >> sh, CP = GEP_NT >> sh;
+    if (R < mm) {
+      const int i = j + 1 + R;
+      const double uji = GM(j, i);
+      for (int c = j + 1 + CQ; c < n; c += CP) GM(i, c) -= uji * GM(j, c);
     }
     __syncthreads();
   }
-  for (int e = tid; e < n * n; e += T) {
-    int i = e % n, j = e / n;
+  for (int e = tid; e < n * n; e += GEP_NT) {
+    const int i = e % n, j = e / n;
     if (i > j) U[e] = 0.0;
   }
-  __syncthreads();
-  if (sflag) {
-    if (tid == 0) atomicOr(err, 4);
-    return;
+  // ---- 2. W = A U^-1 (columns, left to right), then C = U^-T W (rows)
+  for (int e = tid; e < n * n; e += GEP_NT) {
+    const int i = e % n, j = e / n;
+    GM(i, j) = Ain[min(i, j) + (long)max(i, j) * n];
   }
-  // Z = C U^-1 (rows independent)
-  for (int r = tid; r < n; r += T) {
+  if (tid < n) s_u[0][tid] = GU(0, tid);
+  __syncthreads();
+  {
+    const int sh = gep_sh(n);
+    const int R = tid & ((1 << sh) - 1), CQ = tid >> sh, CP = GEP_NT >> sh;
     for (int j = 0; j < n; j++) {
-      double s = C[r + (long)j * ldc];
-      for (int k = 0; k < j; k++) s -= Z[r + (long)k * n] * U[k + (long)j * n];
-      Z[r + (long)j * n] = s / U[j + (long)j * n];
+      const double* ur = s_u[j & 1];   // ur[c] = U(j, c)
+      if (j + 1 < n && tid < n) s_u[(j + 1) & 1][tid] = GU(j + 1, tid);
+      if (R < n) {
+        const double w = GM(R, j) / ur[j];
+        for (int c = j + 1 + CQ; c < n; c += CP) GM(R, c) -= w * ur[c];
+      }
+      __syncthreads();
     }
   }
+  for (int e = tid; e < n * n; e += GEP_NT) {
+    const int i = e % n, j = e / n;
+    GM(i, j) /= GU(j, j);
+  }
+  if (tid < n) s_u[0][tid] = GU(0, tid);
   __syncthreads();
-  // C = U^-T Z (columns independent)
-  for (int c = tid; c < n; c += T) {
-    for (int i = 0; i < n; i++) {
-      double s = Z[i + (long)c * n];
-      for (int k = 0; k < i; k++) s -= U[k + (long)i * n] * C[k + (long)c * ldc];
-      C[i + (long)c * ldc] = s / U[i + (long)i * n];
+  for (int i = 0; i < n; i++) {
+    const double* ur = s_u[i & 1];   // ur[r] = U(i, r)
+    if (i + 1 < n && tid < n) s_u[(i + 1) & 1][tid] = GU(i + 1, tid);
+    const int mm = n - i - 1, sh = gep_sh(mm);
+    const int R = tid & ((1 << sh) - 1), CQ = tid >> sh, CP = GEP_NT >> sh;
+    if (R < mm) {
+      const int r = i + 1 + R;
+      const double uir = ur[r] / ur[i];
+      for (int c = CQ; c < n; c += CP) GM(r, c) -= uir * GM(i, c);
     }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * n; e += GEP_NT) {
+    const int i = e % n, j = e / n;
+    GM(i, j) /= GU(i, i);
   }
   __syncthreads();
-  for (int e = tid; e < n * n; e += T) {
-    int i = e % n, j = e / n;
+  for (int e = tid; e < n * n; e += GEP_NT) {
+    const int i = e % n, j = e / n;
     if (i < j) {
-      double v = 0.5 * (C[i + (long)j * ldc] + C[j + (long)i * ldc]);
-      C[i + (long)j * ldc] = v;
-      C[j + (long)i * ldc] = v;
+      const double v = 0.5 * (GM(i, j) + GM(j, i));
+      GM(i, j) = v;
+      GM(j, i) = v;
     }
-    V[e] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
-  int m = n + (n & 1);  // players (one dummy when n is odd)
-  int np = m / 2;
-  // e / n as a multiply-high by ceil(2^32 / n): exact while e (n - 1) < 2^32,
-  // here e < np n + GEP_U T < 2^15 and n <= 200
-  const unsigned long long ninv = ((1ull << 32) + n - 1) / n;
-  // sweep status, set by thread 0 after each off-diagonal norm:
-  // 0 running, 1 converged, 2 non-finite entries
-  __shared__ int jstat;
-  if (tid == 0) jstat = 0;
-  for (int sweep = 0; sweep <= GEP_SWEEPS; sweep++) {
-    double off = 0.0, tot = 0.0;
-    for (int e = tid; e < n * n; e += T) {
-      const int i = e % n, j = e / n;
-      const double c = C[i + (long)j * ldc];
-      const double v = c * c;
-      tot += v;
-      if (i != j) off += v;
+  // ---- 3. tridiagonalisation.  Step k maps x = M(k+1.., k) (m = n-k-1
+  // entries) to alpha e_1 by H = I - b v v^T and updates the trailing block
+  // B = M(k+1.., k+1..) to H B H; the update also stages the next step's x
+  // (s_v) and the partial sums of its squares past the first (s_part).
+  if (n >= 3) {
+    double sq = 0.0;
+    if (tid < n - 1) {
+      const double x = GM(1 + tid, 0);
+      s_v[0][tid] = x;
+      if (tid >= 1) sq = x * x;
     }
-    // block reduction of (off, tot)
-    for (int o = 32; o > 0; o >>= 1) {
-      off += __shfl_down(off, o, 64);
-      tot += __shfl_down(tot, o, 64);
-    }
-    if ((tid & 63) == 0) {
-      sred[tid >> 6] = off;
-      sred[16 + (tid >> 6)] = tot;
-    }
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
+    if (lane == 0) s_part[0][wv] = sq;
     __syncthreads();
+  }
+  for (int k = 0; k + 2 < n; k++) {
+    const int m = n - k - 1, cb = k & 1, nb = cb ^ 1;
+    const int sh = gep_sh(m), tsh = 10 - sh;   // 2^tsh threads per row
+    const int R = tid & ((1 << sh) - 1), CQ = tid >> sh, CP = GEP_NT >> sh;
+    double sig = 0.0;
+#pragma unroll
+    for (int w = 0; w < GEP_NT / 64; w++) sig += s_part[cb][w];
+    const double x0 = s_v[cb][0];
+    const double mu = sqrt(x0 * x0 + sig);
+    const double v0 = sig == 0.0 ? x0 : (x0 <= 0.0 ? x0 - mu : x0 + mu);
+    const double b = sig == 0.0 ? 0.0 : 2.0 / (v0 * v0 + sig);
     if (tid == 0) {
-      double a = 0.0, b = 0.0;
-      for (int w = 0; w < T / 64; w++) {
-        a += sred[w];
-        b += sred[16 + w];
+      s_d[k] = GM(k, k);
+      s_b[k] = b;
+      s_e[k] = sig == 0.0 ? x0 : (x0 <= 0.0 ? mu : -mu);
+      GM(k + 1, k) = v0;   // column k keeps v for the forming of Q
+    }
+    // p = b B v: 2^tsh threads per row (adjacent lanes), shuffled together
+    {
+      const int r = tid >> tsh, tq = tid & ((1 << tsh) - 1), ts = 1 << tsh;
+      double s = 0.0, s2 = 0.0;
+      if (r < m) {
+        int c = tq;
+        for (; c + ts < m; c += 2 * ts) {
+          s += GM(k + 1 + r, k + 1 + c) * (c == 0 ? v0 : s_v[cb][c]);
+          s2 += GM(k + 1 + r, k + 1 + c + ts) * s_v[cb][c + ts];
+        }
+        if (c < m) s += GM(k + 1 + r, k + 1 + c) * (c == 0 ? v0 : s_v[cb][c]);
       }
-      if (!(isfinite(a) && isfinite(b)))
-        jstat = 2;
-      else if (a <= 1e-30 * b || a == 0.0)
-        jstat = 1;
-      else if (sweep == GEP_SWEEPS && a <= 1e-20 * b)
-        jstat = 1;   // rounding-level residue after the last sweep
+      s += s2;
+      for (int o = 1; o < ts; o <<= 1) s += __shfl_xor(s, o, 64);
+      const double p = b * s;
+      double vp = 0.0;
+      if (tq == 0 && r < m) {
+        s_p[r] = p;
+        vp = (r == 0 ? v0 : s_v[cb][r]) * p;
+      }
+      for (int o = 32; o > 0; o >>= 1) vp += __shfl_xor(vp, o, 64);
+      if (lane == 0) s_q[wv] = vp;
     }
     __syncthreads();
-    if (jstat || sweep == GEP_SWEEPS) break;
-    for (int round = 0; round < m - 1; round++) {
-      // round-robin pairing: player 0 fixed, others rotate
-      for (int q = tid; q < np; q += T) {
-        int a = (q == 0) ? 0 : 1 + (q - 1 + round) % (m - 1);
-        int b = 1 + (m - 2 - q + round) % (m - 1);
-        int p0 = min(a, b), p1 = max(a, b);
-        pairs[2 * q] = p0;
-        pairs[2 * q + 1] = p1;
-        double c = 1.0, s = 0.0;
-        if (p1 < n) {
-          double apq = C[p0 + (long)p1 * ldc];
-          if (apq != 0.0) {
-            double app = C[p0 + (long)p0 * ldc], aqq = C[p1 + (long)p1 * ldc];
-            double th = (aqq - app) / (2.0 * apq);
-            double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-            c = 1.0 / sqrt(t * t + 1.0);
-            s = t * c;
+    // B -= v w^T + w v^T, w = p - (b v^T p / 2) v; each entry as the sum of
+    // two rounded products, so that B stays exactly symmetric
+    {
+      double vtp = 0.0;
+#pragma unroll
+      for (int w = 0; w < GEP_NT / 64; w++) vtp += s_q[w];
+      const double K = 0.5 * b * vtp;
+      double sq = 0.0;
+      if (R < m) {
+        const double vr = R == 0 ? v0 : s_v[cb][R];
+        const double wr = s_p[R] - K * vr;
+        for (int c = CQ; c < m; c += CP) {
+          const double vc = c == 0 ? v0 : s_v[cb][c];
+          const double wc = s_p[c] - K * vc;
+          double nv;
+          {
+#pragma clang fp contract(off)
+            const double t = vr * wc + wr * vc;
+            nv = GM(k + 1 + R, k + 1 + c) - t;
+          }
+          GM(k + 1 + R, k + 1 + c) = nv;
+          // column k+1 below the next diagonal: the next step's x
+          if (c == 0 && R >= 1) {
+            s_v[nb][R - 1] = nv;
+            if (R >= 2) sq = nv * nv;
           }
         }
-        rc[q] = c;
-        rs[q] = s;
       }
-      __syncthreads();
-      // columns of C and V, then rows of C.  Every thread first loads all
-      // of its (up to GEP_U per pass) element pairs -- from offset 0 when the
-      // pair has no rotation -- and only then stores, so its loads are in
-      // flight together instead of one round trip per element.
-      for (int e0 = 0; e0 < np * n; e0 += GEP_U * T) {
-        double cx[GEP_U], cy[GEP_U], vx[GEP_U], vy[GEP_U], cc[GEP_U], ss[GEP_U];
-        long o0[GEP_U], o1[GEP_U], w0[GEP_U], w1[GEP_U];
-        bool live[GEP_U];
-#pragma unroll
-        for (int u = 0; u < GEP_U; u++) {
-          const int e = e0 + u * T + tid;
-          const int qe = (int)(((unsigned long long)e * ninv) >> 32);
-          const int q = e < np * n ? qe : 0, r = e - qe * n;
-          const int p0 = pairs[2 * q], p1 = pairs[2 * q + 1];
-          live[u] = e < np * n && p1 < n && rs[q] != 0.0;
-          o0[u] = live[u] ? r + (long)p0 * ldc : 0;
-          o1[u] = live[u] ? r + (long)p1 * ldc : 0;
-          w0[u] = live[u] ? r + (long)p0 * n : 0;
-          w1[u] = live[u] ? r + (long)p1 * n : 0;
-          cc[u] = rc[q];
-          ss[u] = rs[q];
-          cx[u] = C[o0[u]];
-          cy[u] = C[o1[u]];
-          vx[u] = V[w0[u]];
-          vy[u] = V[w1[u]];
+      for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
+      if (lane == 0) s_part[nb][wv] = sq;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (n >= 2) {
+      s_d[n - 2] = GM(n - 2, n - 2);
+      s_e[n - 2] = GM(n - 1, n - 2);
+    }
+    s_d[n - 1] = GM(n - 1, n - 1);
+    s_e[n - 1] = 0.0;
+    GM(n - 1, n - 1) = 1.0;
+  }
+  __syncthreads();
+  // Q = H_0 H_1 ... H_{n-3}, formed from the last reflection back: before
+  // step k the block (k+2.., k+2..) holds H_{k+1} ... H_{n-3}; row and
+  // column k+1 become e_{k+1} (column k+1 below the diagonal held v_{k+1},
+  // no longer needed), then H_k applies from the left.
+  for (int k = n - 3; k >= 0; k--) {
+    const int j1 = k + 1, m = n - j1;
+    const int sh = gep_sh(m), tsh = 10 - sh;
+    const int R = tid & ((1 << sh) - 1), CQ = tid >> sh, CP = GEP_NT >> sh;
+    const double b = s_b[k];
+    for (int i = j1 + 1 + tid; i < n; i += GEP_NT) {
+      GM(i, j1) = 0.0;
+      GM(j1, i) = 0.0;
+    }
+    if (tid == 0) GM(j1, j1) = 1.0;
+    // y_j = b sum_i v_i Q(i, j): column j1 of Q is e_{j1}, row j1 is zero
+    {
+      const int jj = tid >> tsh, tq = tid & ((1 << tsh) - 1), ts = 1 << tsh;
+      double s = 0.0, s2 = 0.0;
+      if (jj < m && jj > 0) {
+        int i = j1 + 1 + tq;
+        for (; i + ts < n; i += 2 * ts) {
+          s += GM(i, k) * GM(i, j1 + jj);
+          s2 += GM(i + ts, k) * GM(i + ts, j1 + jj);
         }
-#pragma unroll
-        for (int u = 0; u < GEP_U; u++)
-          if (live[u]) {
-            C[o0[u]] = cc[u] * cx[u] - ss[u] * cy[u];
-            C[o1[u]] = ss[u] * cx[u] + cc[u] * cy[u];
-            V[w0[u]] = cc[u] * vx[u] - ss[u] * vy[u];
-            V[w1[u]] = ss[u] * vx[u] + cc[u] * vy[u];
-          }
+        if (i < n) s += GM(i, k) * GM(i, j1 + jj);
       }
-      __syncthreads();
-      for (int e0 = 0; e0 < np * n; e0 += GEP_U * T) {
-        double cx[GEP_U], cy[GEP_U], cc[GEP_U], ss[GEP_U];
-        long o0[GEP_U], o1[GEP_U];
-        bool live[GEP_U], zr0[GEP_U], zr1[GEP_U];
-#pragma unroll
-        for (int u = 0; u < GEP_U; u++) {
-          const int e = e0 + u * T + tid;
-          const int qe = (int)(((unsigned long long)e * ninv) >> 32);
-          const int q = e < np * n ? qe : 0, r = e - qe * n;
-          const int p0 = pairs[2 * q], p1 = pairs[2 * q + 1];
-          live[u] = e < np * n && p1 < n && rs[q] != 0.0;
-          zr0[u] = r == p1;
-          zr1[u] = r == p0;
-          o0[u] = live[u] ? p0 + (long)r * ldc : 0;
-          o1[u] = live[u] ? p1 + (long)r * ldc : 0;
-          cc[u] = rc[q];
-          ss[u] = rs[q];
-          cx[u] = C[o0[u]];
-          cy[u] = C[o1[u]];
+      s += s2;
+      for (int o = 1; o < ts; o <<= 1) s += __shfl_xor(s, o, 64);
+      if (tq == 0 && jj < m) s_p[jj] = jj == 0 ? b * GM(j1, k) : b * s;
+    }
+    __syncthreads();
+    if (R < m) {
+      const double vi = GM(j1 + R, k);
+      for (int c = CQ; c < m; c += CP) GM(j1 + R, j1 + c) -= vi * s_p[c];
+    }
+    __syncthreads();
+  }
+  for (int i = 1 + tid; i < n; i += GEP_NT) {
+    GM(i, 0) = 0.0;
+    GM(0, i) = 0.0;
+  }
+  if (tid == 0) {
+    GM(0, 0) = 1.0;
+    s_flag[0] = 0;
+    s_flag[1] = 0;
+  }
+  __syncthreads();
+  // ---- 4. QL.  Round t: wave 0 forms the next sweep's rotations into
+  // buffer t & 1 (range s_lohi, lo > hi for none) or flags the end (every
+  // lane computes the same chain; lane 0 stores); the threads 64 + row,
+  // row < n, apply buffer (t - 1) & 1 to row `row` of Q.
+  {
+    int l = 0, iter = 0;   // wave 0's state
+    for (int t = 0;; t++) {
+      const int pb = t & 1;
+      if (wv == 0) {
+        int lo = 1, hi = 0, fin = 0;
+        for (;;) {
+          if (l >= n) {
+            fin = 1;
+            break;
+          }
+          // first m >= l with a negligible e(m), 64 candidates at a time
+          int m = n - 1;
+          for (int base = l; base < n - 1; base += 64) {
+            const int mc = base + lane;
+            bool neg = false;
+            if (mc < n - 1) {
+              const double dd = fabs(s_d[mc]) + fabs(s_d[mc + 1]);
+              neg = fabs(s_e[mc]) <= 2.220446049250313e-16 * dd;
+            }
+            const unsigned long long bal = __ballot(neg);
+            if (bal) {
+              m = base + __ffsll((long long)bal) - 1;
+              break;
+            }
+          }
+          if (m == l) {
+            l++;
+            iter = 0;
+            continue;
+          }
+          if (++iter > GEP_QL_ITER) {
+            fin = 2;
+            break;
+          }
+          const double dl = s_d[l], el = s_e[l];
+          double g = (s_d[l + 1] - dl) / (2.0 * el);
+          double r = hypot(g, 1.0);
+          g = s_d[m] - dl + el / (g + (g >= 0.0 ? r : -r));
+          double s = 1.0, c = 1.0, p = 0.0;
+          int i = m - 1;
+          bool under = false;
+          // operands of step i: e(i), d(i), d(i+1), none written before it
+          double ei = s_e[i], di = s_d[i], di1 = s_d[i + 1];
+          for (; i >= l; i--) {
+            const double en = s_e[max(i - 1, 0)], dn = s_d[max(i - 1, 0)];
+            const double f = s * ei, bb = c * ei;
+            const double r2 = f * f + g * g;
+            if (r2 == 0.0) {
+              if (lane == 0) {
+                s_e[i + 1] = 0.0;
+                s_d[i + 1] = di1 - p;
+                s_e[m] = 0.0;
+              }
+              under = true;
+              break;
+            }
+            double rinv;
+            if (isfinite(r2)) {
+              rinv = rsqrt(r2);
+              r = r2 * rinv;
+            } else {
+              r = hypot(f, g);
+              rinv = 1.0 / r;
+            }
+            s = f * rinv;
+            c = g * rinv;
+            g = di1 - p;
+            const double qv = (di - g) * s + 2.0 * c * bb;
+            p = s * qv;
+            if (lane == 0) {
+              s_e[i + 1] = r;
+              s_d[i + 1] = g + p;
+              s_rc[pb][i] = c;
+              s_rs[pb][i] = s;
+            }
+            g = c * qv - bb;
+            ei = en;
+            di1 = di;
+            di = dn;
+          }
+          if (!under) {
+            if (lane == 0) {
+              s_d[l] = dl - p;
+              s_e[l] = g;
+              s_e[m] = 0.0;
+            }
+            lo = l;
+          } else {
+            lo = i + 1;
+          }
+          hi = m - 1;
+          break;
         }
-#pragma unroll
-        for (int u = 0; u < GEP_U; u++)
-          if (live[u]) {
-            // the pair's own entries (p0, p1) and (p1, p0) are annihilated
-            // by the rotation: stored as exact zeros
-            C[o0[u]] = zr0[u] ? 0.0 : cc[u] * cx[u] - ss[u] * cy[u];
-            C[o1[u]] = zr1[u] ? 0.0 : ss[u] * cx[u] + cc[u] * cy[u];
+        if (lane == 0) {
+          s_lohi[pb][0] = lo;
+          s_lohi[pb][1] = hi;
+          if (fin) s_flag[0] = fin;
+        }
+      } else if (t >= 1 && tid < 64 + n) {
+        const int row = tid - 64, ob = pb ^ 1;
+        const int lo = s_lohi[ob][0], hi = s_lohi[ob][1];
+        if (lo <= hi) {
+          const double* rc = s_rc[ob];
+          const double* rs = s_rs[ob];
+          double f = GM(row, hi + 1);
+          int i = hi;
+          for (; i - 3 >= lo; i -= 4) {
+            const double z0 = GM(row, i), z1 = GM(row, i - 1);
+            const double z2 = GM(row, i - 2), z3 = GM(row, i - 3);
+            GM(row, i + 1) = rs[i] * z0 + rc[i] * f;
+            f = rc[i] * z0 - rs[i] * f;
+            GM(row, i) = rs[i - 1] * z1 + rc[i - 1] * f;
+            f = rc[i - 1] * z1 - rs[i - 1] * f;
+            GM(row, i - 1) = rs[i - 2] * z2 + rc[i - 2] * f;
+            f = rc[i - 2] * z2 - rs[i - 2] * f;
+            GM(row, i - 2) = rs[i - 3] * z3 + rc[i - 3] * f;
+            f = rc[i - 3] * z3 - rs[i - 3] * f;
           }
+          for (; i >= lo; i--) {
+            const double zi = GM(row, i);
+            GM(row, i + 1) = rs[i] * zi + rc[i] * f;
+            f = rc[i] * zi - rs[i] * f;
+          }
+          GM(row, lo) = f;
+        }
       }
       __syncthreads();
+      if (s_flag[0]) break;
     }
   }
-  // not converged or not finite: error bit 8, outputs untouched
-  if (jstat != 1) {
+  // ---- 5. errors, sort, back-transformation
+  if (tid < n && !isfinite(s_d[tid])) s_flag[1] = 1;
+  __syncthreads();
+  if (s_flag[0] != 1 || s_flag[1]) {
     if (tid == 0) atomicOr(err, 8);
     return;
   }
-  // ascending order (rank sort, ties by index)
-  for (int i = tid; i < n; i += T) {
-    double wi = C[i + (long)i * ldc];
+  if (tid < n) {   // rank sort, ties by index
+    const double wi = s_d[tid];
     int r = 0;
     for (int j = 0; j < n; j++) {
-      double wj = C[j + (long)j * ldc];
-      if (wj < wi || (wj == wi && j < i)) r++;
+      const double wj = s_d[j];
+      if (wj < wi || (wj == wi && j < tid)) r++;
     }
-    ord[r] = i;
-    wt[r] = wi;
+    s_ord[r] = tid;
+  }
+  // X = U^-1 Z: rows from the last; column i of U staged (U(r, i), r <= i)
+  if (tid < n) s_u[(n - 1) & 1][tid] = GU(tid, n - 1);
+  __syncthreads();
+  for (int i = n - 1; i >= 0; i--) {
+    const double* uc = s_u[i & 1];
+    if (i >= 1 && tid < n) s_u[(i - 1) & 1][tid] = GU(tid, i - 1);
+    const int sh = gep_sh(i);
+    const int R = tid & ((1 << sh) - 1), CQ = tid >> sh, CP = GEP_NT >> sh;
+    if (R < i) {
+      const double uri = uc[R] / uc[i];
+      for (int c = CQ; c < n; c += CP) GM(R, c) -= uri * GM(i, c);
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * n; e += GEP_NT) {
+    const int i = e % n, j = e / n;
+    GM(i, j) /= GU(i, i);
   }
   __syncthreads();
-  // X = U^-1 V (columns independent), sign convention
-  for (int c = tid; c < n; c += T) {
-    const double* v = V + (long)ord[c] * n;
-    double* x = Xo + (long)c * n;
-    for (int i = n - 1; i >= 0; i--) {
-      double s = v[i];
-      for (int k = i + 1; k < n; k++) s -= U[i + (long)k * n] * x[k];
-      x[i] = s / U[i + (long)i * n];
-    }
+  // sign of each output column (largest |x| positive, first of ties)
+  if (tid < n) {
+    const int src = s_ord[tid];
     int pm = 0;
     for (int i = 1; i < n; i++)
-      if (fabs(x[i]) > fabs(x[pm])) pm = i;
-    if (x[pm] < 0)
-      for (int i = 0; i < n; i++) x[i] = -x[i];
-    wo[c] = wt[c];
+      if (fabs(GM(i, src)) > fabs(GM(pm, src))) pm = i;
+    s_p[tid] = GM(pm, src) < 0.0 ? -1.0 : 1.0;
+    wo[tid] = s_d[src];
   }
+  __syncthreads();
+  for (int e = tid; e < n * n; e += GEP_NT) {
+    const int i = e % n, c = e / n;
+    Xo[e] = s_p[c] * GM(i, s_ord[c]);
+  }
+#undef GM
+#undef GU
 }
 
 // --------------------------------------------------------------------------

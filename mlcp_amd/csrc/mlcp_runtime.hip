@@ -148,8 +148,7 @@ static int setup_kernels() {
                       NX_KERNELS2(k_op_solve, 3),  NX_KERNELS2(k_op_solve, 4),
                       (const void*)k_lc_solve<1>,  (const void*)k_lc_solve<2>,
                       (const void*)k_lc_solve<3>,  (const void*)k_lc_solve<4>,
-                      (const void*)k_gep<1>,
-                      (const void*)k_gep<2>};
+                      (const void*)k_gep<1>};
 #undef NX_KERNELS2
 #undef NX_KERNELS
   g_lds_dev = (size_t)v;
@@ -719,8 +718,8 @@ static int check_err(mlcp_session* s, const char* phase) {
     if (e & 1) m += "NaN or zero norm on update; ";
     if (e & 2) m += "non-positive pivot in the ALS normal equations; ";
     if (e & 4) m += "overlap matrix not positive definite; ";
-    if (e & 8) m += "block eigensolver: non-finite matrix or no convergence "
-                    "within the sweep limit; ";
+    if (e & 8) m += "block eigensolver: non-finite matrix or no QL "
+                    "convergence; ";
     return fail(m);
   }
   return 0;
@@ -1023,10 +1022,8 @@ static int launch_ortho(mlcp_session* s, int nals) {
 // LDS placement of the eigensolver's work matrices (k_gep MODE) for order n
 // and the dynamic LDS it needs.
 static int gep_mode(int n, size_t* lds) {
-  // C with the odd leading dimension n | 1, V with n
+  // the working matrix in LDS with the odd leading dimension n | 1
   const size_t mc = (size_t)n * (n | 1) * sizeof(double);
-  const size_t mv = (size_t)n * n * sizeof(double);
-  if (mc + mv <= lds_limit((const void*)k_gep<2>)) { *lds = mc + mv; return 2; }
   if (mc <= lds_limit((const void*)k_gep<1>)) { *lds = mc; return 1; }
   *lds = 0;
   return 0;
@@ -1036,9 +1033,7 @@ static int gep_mode(int n, size_t* lds) {
   do {                                                                    \
     size_t lds_;                                                          \
     const int mode_ = gep_mode(n, &lds_);                                 \
-    if (mode_ == 2)                                                       \
-      PLAUNCH(kid, k_gep<2>, dim3(1), dim3(1024), lds_, __VA_ARGS__);     \
-    else if (mode_ == 1)                                                  \
+    if (mode_ == 1)                                                       \
       PLAUNCH(kid, k_gep<1>, dim3(1), dim3(1024), lds_, __VA_ARGS__);     \
     else                                                                  \
       PLAUNCH(kid, k_gep<0>, dim3(1), dim3(1024), lds_, __VA_ARGS__);     \
@@ -2113,10 +2108,7 @@ extern "C" int mlcp_hip_gep(int n, const double* A, const double* Bm,
   if (setup_kernels()) return -1;
   size_t lds = 0;
   const int mode = gep_mode(n, &lds);
-  if (mode == 2)
-    hipLaunchKernelGGL(k_gep<2>, dim3(1), dim3(1024), lds, 0, dA, dB, n, ws,
-                       dX, dw, e.p);
-  else if (mode == 1)
+  if (mode == 1)
     hipLaunchKernelGGL(k_gep<1>, dim3(1), dim3(1024), lds, 0, dA, dB, n, ws,
                        dX, dw, e.p);
   else
@@ -2126,8 +2118,7 @@ extern "C" int mlcp_hip_gep(int n, const double* A, const double* Bm,
   int err = 0;
   HCHK(hipMemcpy(&err, e.p, sizeof(int), hipMemcpyDeviceToHost));
   if (err & 4) return fail("gep: overlap matrix not positive definite");
-  if (err) return fail("gep: non-finite matrix or no convergence within the "
-                       "sweep limit");
+  if (err) return fail("gep: non-finite matrix or no QL convergence");
   HCHK(hipMemcpy(X, dX, nn * sizeof(double), hipMemcpyDeviceToHost));
   HCHK(hipMemcpy(w, dw, n * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
@@ -2141,7 +2132,7 @@ extern "C" int mlcp_hip_gep(int n, const double* A, const double* Bm,
 // array's name is only a stand-in: its device address on the current GPU is
 // read once per device with hipGetSymbolAddress and used for every launch.
 #define GEP_NA_NMAX MLCP_BMAX
-#define GEP_NA_WORDS (4 * GEP_NA_NMAX * GEP_NA_NMAX + 2 * GEP_NA_NMAX + 8)
+#define GEP_NA_WORDS (2 * GEP_NA_NMAX * GEP_NA_NMAX)   // U and the working matrix
 static __device__ __align__(256) double scratch[GEP_NA_WORDS];
 #define GEP_NA_MAXDEV 64
 static double* g_scratch_dev[GEP_NA_MAXDEV];
@@ -2178,10 +2169,7 @@ extern "C" int mlcp_hip_gep_noalloc(int n, const double* dA, const double* dB,
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, dinfo, 1L, 0);
   size_t lds = 0;
   const int mode = gep_mode(n, &lds);
-  if (mode == 2)
-    hipLaunchKernelGGL(k_gep<2>, dim3(1), dim3(1024), lds, st, dA, dB, n, ws,
-                       dX, dw, dinfo);
-  else if (mode == 1)
+  if (mode == 1)
     hipLaunchKernelGGL(k_gep<1>, dim3(1), dim3(1024), lds, st, dA, dB, n, ws,
                        dX, dw, dinfo);
   else

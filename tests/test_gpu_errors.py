@@ -89,7 +89,7 @@ def test_rank_deficient_block_is_regularised_like_the_oracle(hip):
 
 
 def test_eigensolver_nan_matrix(hip):
-    """A NaN in QHQ with a positive definite S: the Jacobi sweeps cannot
+    """A NaN in QHQ with a positive definite S: the QL iteration cannot
     converge; the eigensolver reports bit 8 instead of returning NaNs."""
     L = hip
     say("gep NaN")
