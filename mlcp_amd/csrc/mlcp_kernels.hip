@@ -1748,10 +1748,12 @@ __global__ __launch_bounds__(MLCP_NT, RTM < 4 ? 4 : 2) void k_lc_refresh(LcArgs 
 }
 
 // partial right-hand side of target t over one chunk of old columns:
-// part(i,m) = sum_c w_c cO_c prod_{d!=k} X_d(i,c) O_c^k(m)
+// part(i,m) = sum_c w_c cO_c prod_{d!=k} X_d(i,c) O_c^k(m).  Targets of rank
+// <= 16 RTM.
+template <int RTM>
 __global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
                                                      const int2* jobs, int k) {
-  __shared__ double P[LC_CHUNK * MLCP_RMAX];   // P(c, i), c-major
+  __shared__ double P[LC_CHUNK * 16 * RTM];   // P(c, i), c-major
   int2 jb = jobs[blockIdx.x];
   if (a.done && a.done[jb.x]) return;   // converged target (tolerance form)
   int t = jb.x, ch = jb.y, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof;
@@ -1760,9 +1762,12 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
   long xs = (long)a.xr * a.ncold;
   const double* X = a.X + (long)t * D * xs;
   // P(c, i) = w_c coef_c prod_{m != k} X_m(i, c): every thread owns up to
-  // LC_CHUNK * MLCP_RMAX / MLCP_NT entries, mode-outer so the loads of one
-  // mode are all in flight together
-  constexpr int PE = LC_CHUNK * MLCP_RMAX / MLCP_NT;
+  // PE entries; the loads of MG modes are in flight together (MG PE <= 32
+  // doubles of registers), the products taken mode by mode in order.  An
+  // entry past the chunk reads X at offset (c0, 0), in bounds, and is not
+  // stored.
+  constexpr int PE = LC_CHUNK * 16 * RTM / MLCP_NT;
+  constexpr int MG = RTM <= 1 ? 8 : (RTM == 2 ? 4 : 2);
   double pr[PE];
   long off[PE];
 #pragma unroll
@@ -1773,14 +1778,20 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
     off[j] = (long)(c0 + c) * a.xr + i;
     pr[j] = ok ? w[c0 + c] * a.p.coef[a.ocol0 + c0 + c] : 0.0;
   }
-  for (int m = 1; m < D; m++) {
-    const double* Xm = X + (long)((k + m) % D) * xs;
-    double v[PE];
+  for (int m0 = 1; m0 < D; m0 += MG) {
+    double v[MG][PE];
 #pragma unroll
-    for (int j = 0; j < PE; j++)
-      v[j] = (TID + j * MLCP_NT < nc * R) ? Xm[off[j]] : 0.0;
+    for (int g = 0; g < MG; g++) {
+      const double* Xm = X + (long)((k + min(m0 + g, D - 1)) % D) * xs;
 #pragma unroll
-    for (int j = 0; j < PE; j++) pr[j] *= v[j];
+      for (int j = 0; j < PE; j++) v[g][j] = Xm[off[j]];
+    }
+#pragma unroll
+    for (int g = 0; g < MG; g++)
+      if (m0 + g < D) {
+#pragma unroll
+        for (int j = 0; j < PE; j++) pr[j] *= v[g][j];
+      }
   }
 #pragma unroll
   for (int j = 0; j < PE; j++) {

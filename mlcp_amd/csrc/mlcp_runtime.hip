@@ -1110,8 +1110,18 @@ static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
   }
   for (int it = 0; it < nitn; it++)
     for (int k = 0; k < D; k++) {
-      PLAUNCH(KID_LC_RHS, k_lc_rhs, dim3(nj), dim3(MLCP_NT), 0, a,
-              (const int2*)s->lcJobs.p, k);
+      if (s->Rmax <= 16)
+        PLAUNCH(KID_LC_RHS, k_lc_rhs<1>, dim3(nj), dim3(MLCP_NT), 0, a,
+                (const int2*)s->lcJobs.p, k);
+      else if (s->Rmax <= 32)
+        PLAUNCH(KID_LC_RHS, k_lc_rhs<2>, dim3(nj), dim3(MLCP_NT), 0, a,
+                (const int2*)s->lcJobs.p, k);
+      else if (s->Rmax <= 48)
+        PLAUNCH(KID_LC_RHS, k_lc_rhs<3>, dim3(nj), dim3(MLCP_NT), 0, a,
+                (const int2*)s->lcJobs.p, k);
+      else
+        PLAUNCH(KID_LC_RHS, k_lc_rhs<4>, dim3(nj), dim3(MLCP_NT), 0, a,
+                (const int2*)s->lcJobs.p, k);
       size_t lds_s = ((size_t)s->Rmax * (s->nbas[k] | 1) +
                       (size_t)s->Rmax * s->Rmax) * sizeof(double);
       if (s->Rmax <= 16)
