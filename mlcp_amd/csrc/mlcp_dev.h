@@ -114,9 +114,10 @@ struct LcArgs {
   int rstr;
   int xr;                // row stride of X (max rank of the targets)
   int nchunk;            // old-column chunks of LC_CHUNK
+  int ngrp;              // groups of LC_GRP chunks: partial sums per target
   double* X;             // [B][ndof][xr * ncold]  X(i,c) = <F_i^d, O_c^d>
   double* G;             // [B][ndof][rstr]
-  double* part;          // [B][nchunk][Rmax * nmax] partial right-hand sides
+  double* part;          // [B][ngrp][Rmax * nmax] partial right-hand sides
   int pstr;              // Rmax * nmax
   int* err;
   // ALS to a fit-residual tolerance, as in OpArgs (per target)
@@ -126,6 +127,7 @@ struct LcArgs {
 };
 
 #define LC_CHUNK 64
+#define LC_GRP 4      // chunks of one k_lc_rhs workgroup
 #define GS_ICH 16                                    // columns per GS job
 #define GS_PCS ((MLCP_RMAX + GS_ICH - 1) / GS_ICH)   // partial sums per state
 

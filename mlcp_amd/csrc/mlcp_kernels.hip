@@ -1747,91 +1747,110 @@ __global__ __launch_bounds__(MLCP_NT, RTM < 4 ? 4 : 2) void k_lc_refresh(LcArgs 
   }
 }
 
-// partial right-hand side of target t over one chunk of old columns:
-// part(i,m) = sum_c w_c cO_c prod_{d!=k} X_d(i,c) O_c^k(m).  Targets of rank
-// <= 16 RTM.
+// partial right-hand side of target t over one group of LC_GRP chunks of
+// old columns: part(i,m) = sum_c w_c cO_c prod_{d!=k} X_d(i,c) O_c^k(m),
+// summed over the group's chunks in the MFMA accumulators, so that each
+// group writes one partial instead of one per chunk.  Workgroup = (target
+// t, group).  Targets of rank <= 16 RTM.
 template <int RTM>
-__global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a,
-                                                     const int2* jobs, int k) {
+__global__ __launch_bounds__(MLCP_NT) void k_lc_rhs(LcArgs a, int k) {
   __shared__ double P[LC_CHUNK * 16 * RTM];   // P(c, i), c-major
-  int2 jb = jobs[blockIdx.x];
-  if (a.done && a.done[jb.x]) return;   // converged target (tolerance form)
-  int t = jb.x, ch = jb.y, R = a.R[t], n = a.p.nbas[k], D = a.p.ndof;
-  int c0 = ch * LC_CHUNK, nc = min(LC_CHUNK, a.ncold - c0);
+  const int t = blockIdx.x / a.ngrp, grp = blockIdx.x - t * a.ngrp;
+  if (a.done && a.done[t]) return;   // converged target (tolerance form)
+  const int R = a.R[t], n = a.p.nbas[k], D = a.p.ndof;
+  const int chb = grp * LC_GRP, che = min(a.nchunk, chb + LC_GRP);
   const double* w = a.wcol + (long)t * a.ncold;
-  long xs = (long)a.xr * a.ncold;
+  const long xs = (long)a.xr * a.ncold;
   const double* X = a.X + (long)t * D * xs;
-  // P(c, i) = w_c coef_c prod_{m != k} X_m(i, c): every thread owns up to
-  // PE entries; the loads of MG modes are in flight together (MG PE <= 32
-  // doubles of registers), the products taken mode by mode in order.  An
-  // entry past the chunk reads X at offset (c0, 0), in bounds, and is not
-  // stored.
-  constexpr int PE = LC_CHUNK * 16 * RTM / MLCP_NT;
-  constexpr int MG = RTM <= 1 ? 8 : (RTM == 2 ? 4 : 2);
-  double pr[PE];
-  long off[PE];
-#pragma unroll
-  for (int j = 0; j < PE; j++) {
-    int e = TID + j * MLCP_NT;
-    bool ok = e < nc * R;
-    int i = ok ? e % R : 0, c = ok ? e / R : 0;
-    off[j] = (long)(c0 + c) * a.xr + i;
-    pr[j] = ok ? w[c0 + c] * a.p.coef[a.ocol0 + c0 + c] : 0.0;
-  }
-  for (int m0 = 1; m0 < D; m0 += MG) {
-    double v[MG][PE];
-#pragma unroll
-    for (int g = 0; g < MG; g++) {
-      const double* Xm = X + (long)((k + min(m0 + g, D - 1)) % D) * xs;
-#pragma unroll
-      for (int j = 0; j < PE; j++) v[g][j] = Xm[off[j]];
-    }
-#pragma unroll
-    for (int g = 0; g < MG; g++)
-      if (m0 + g < D) {
-#pragma unroll
-        for (int j = 0; j < PE; j++) pr[j] *= v[g][j];
-      }
-  }
-#pragma unroll
-  for (int j = 0; j < PE; j++) {
-    int e = TID + j * MLCP_NT;
-    if (e < nc * R) P[e] = pr[j];
-  }
-  __syncthreads();
-  // part(i, m) = sum_c P(c, i) O_c^k(m) on the FP64 matrix cores: tile q of
-  // the (R/16) x (n/16) output grid goes to wave q % 4; a k-step is four
-  // old columns (P rows past nc are never read: the operand is zero)
-  double* part = a.part + ((long)t * a.nchunk + ch) * a.pstr;
   const int lane = TID & 63, lr = lane & 15, lq = lane >> 4;
   const int g = __builtin_amdgcn_readfirstlane(TID >> 6);
-  const int RT = (R + 15) >> 4, CT = (n + 15) >> 4, nks = (nc + 3) >> 2;
-  for (int q = g; q < RT * CT; q += 4) {
-    const int rt = q % RT, ct = q / RT;
-    const int i = 16 * rt + lr, mcol = 16 * ct + lr;
-    const bool iok = i < R, mok = mcol < n;
-    // operands from clamped rows and columns (finite, weight zero past nc,
-    // R and n), so every load of a batch is in flight together and the
-    // MFMAs need no guard; two accumulators
-    v4d acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-    const int ic = min(i, R - 1), mc = min(mcol, n - 1);
-    for (int ks0 = 0; ks0 < nks; ks0 += 8) {
-      double av[8], bv[8];
+  const int RT = (R + 15) >> 4, CT = (n + 15) >> 4;
+  // output tile q = g + 4 tq of the RT x CT grid (CT <= 8) in acct[tq]
+  constexpr int TPW = 2 * RTM;
+  v4d acct[TPW];
 #pragma unroll
-      for (int b = 0; b < 8; b++) {
-        const int c = 4 * (ks0 + b) + lq, cc = min(c, nc - 1);
-        av[b] = ld_or0(P + cc * R + ic, c < nc && iok);
-        bv[b] = ld_or0(colp(a.p, k, a.ocol0 + c0 + cc) + mc, c < nc && mok);
+  for (int tq = 0; tq < TPW; tq++) acct[tq] = (v4d){0.0, 0.0, 0.0, 0.0};
+  for (int ch = chb; ch < che; ch++) {
+    const int c0 = ch * LC_CHUNK, nc = min(LC_CHUNK, a.ncold - c0);
+    // P(c, i) = w_c coef_c prod_{m != k} X_m(i, c): every thread owns up to
+    // PE entries; the loads of MG modes are in flight together (MG PE <= 16
+    // doubles of registers, beside the accumulators), the products taken
+    // mode by mode in order.  An entry past the chunk reads X at offset
+    // (c0, 0), in bounds, and is not stored.
+    constexpr int PE = LC_CHUNK * 16 * RTM / MLCP_NT;
+    constexpr int MG = RTM <= 1 ? 4 : (RTM == 2 ? 2 : 1);
+    double pr[PE];
+    int off[PE];
+#pragma unroll
+    for (int j = 0; j < PE; j++) {
+      const int e = TID + j * MLCP_NT;
+      const bool ok = e < nc * R;
+      const int i = ok ? e % R : 0, c = ok ? e / R : 0;
+      off[j] = (c0 + c) * a.xr + i;
+      pr[j] = ok ? w[c0 + c] * a.p.coef[a.ocol0 + c0 + c] : 0.0;
+    }
+    for (int m0 = 1; m0 < D; m0 += MG) {
+      double v[MG][PE];
+#pragma unroll
+      for (int gm = 0; gm < MG; gm++) {
+        const double* Xm = X + (long)((k + min(m0 + gm, D - 1)) % D) * xs;
+#pragma unroll
+        for (int j = 0; j < PE; j++) v[gm][j] = Xm[off[j]];
       }
 #pragma unroll
-      for (int b = 0; b < 8; b++) acc[b & 1] = mfma_f64_4(av[b], bv[b], acc[b & 1]);
-    }
-    const v4d accs = acc[0] + acc[1];
-    // element v of lane l: row 16rt + l/16 + 4v, column 16ct + l%16
+      for (int gm = 0; gm < MG; gm++)
+        if (m0 + gm < D) {
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int io = 16 * rt + lq + 4 * v;
-      if (io < R && mok) part[io * n + mcol] = accs[v];
+          for (int j = 0; j < PE; j++) pr[j] *= v[gm][j];
+        }
+    }
+    if (ch > chb) __syncthreads();   // the previous chunk's P is read
+#pragma unroll
+    for (int j = 0; j < PE; j++) {
+      const int e = TID + j * MLCP_NT;
+      if (e < nc * R) P[e] = pr[j];
+    }
+    __syncthreads();
+    // += sum_c P(c, i) O_c^k(m) on the FP64 matrix cores; a k-step is four
+    // old columns (P rows past nc are never read: the operand is zero)
+    const int nks = (nc + 3) >> 2;
+#pragma unroll
+    for (int tq = 0; tq < TPW; tq++) {
+      const int q = g + 4 * tq;
+      if (q < RT * CT) {
+        const int rt = q % RT, ct = q / RT;
+        const int i = 16 * rt + lr, mcol = 16 * ct + lr;
+        const bool iok = i < R, mok = mcol < n;
+        // operands from clamped rows and columns (finite, weight zero past
+        // nc, R and n), so every load of a batch is in flight together and
+        // the MFMAs need no guard
+        const int ic = min(i, R - 1), mc = min(mcol, n - 1);
+        for (int ks0 = 0; ks0 < nks; ks0 += 4) {
+          double av[4], bv[4];
+#pragma unroll
+          for (int b = 0; b < 4; b++) {
+            const int c = 4 * (ks0 + b) + lq, cc = min(c, nc - 1);
+            av[b] = ld_or0(P + cc * R + ic, c < nc && iok);
+            bv[b] = ld_or0(colp(a.p, k, a.ocol0 + c0 + cc) + mc, c < nc && mok);
+          }
+#pragma unroll
+          for (int b = 0; b < 4; b++) acct[tq] = mfma_f64_4(av[b], bv[b], acct[tq]);
+        }
+      }
+    }
+  }
+  double* part = a.part + ((long)t * a.ngrp + grp) * a.pstr;
+#pragma unroll
+  for (int tq = 0; tq < TPW; tq++) {
+    const int q = g + 4 * tq;
+    if (q < RT * CT) {
+      const int rt = q % RT, ct = q / RT, mcol = 16 * ct + lr;
+      // element v of lane l: row 16rt + l/16 + 4v, column 16ct + l%16
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int io = 16 * rt + lq + 4 * v;
+        if (io < R && mcol < n) part[io * n + mcol] = acct[tq][v];
+      }
     }
   }
 }
@@ -1884,7 +1903,7 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
     gj_factor<RTM>(w, R, Es, fsc);
   }
   {
-    const double* pt = a.part + (long)t * a.nchunk * a.pstr;
+    const double* pt = a.part + (long)t * a.ngrp * a.pstr;
     for (;;) {
       int base = 0;
       if (lane == 0) base = atomicAdd(&next, 64);
@@ -1896,12 +1915,12 @@ __global__ __launch_bounds__(MLCP_NT) void k_lc_solve(LcArgs a, int k) {
       double sv[U];
 #pragma unroll
       for (int u = 0; u < U; u++) sv[u] = 0.0;
-      for (int ch = 0; ch < a.nchunk; ch += U) {
+      for (int ch = 0; ch < a.ngrp; ch += U) {
         double pv[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) pv[u] = pt[(long)min(ch + u, a.nchunk - 1) * a.pstr + e];
+        for (int u = 0; u < U; u++) pv[u] = pt[(long)min(ch + u, a.ngrp - 1) * a.pstr + e];
 #pragma unroll
-        for (int u = 0; u < U; u++) sv[u] += ch + u < a.nchunk ? pv[u] : 0.0;
+        for (int u = 0; u < U; u++) sv[u] += ch + u < a.ngrp ? pv[u] : 0.0;
       }
 #pragma unroll
       for (int h = U / 2; h > 0; h >>= 1)

@@ -1067,6 +1067,7 @@ static LcArgs lc_args(mlcp_session* s, int B, const int* col, const int* R,
   a.rstr = s->Rmax * s->Rmax;
   a.xr = xr;
   a.nchunk = (ncold + LC_CHUNK - 1) / LC_CHUNK;
+  a.ngrp = (a.nchunk + LC_GRP - 1) / LC_GRP;
   a.X = s->lcX.p;
   a.G = s->lcG.p;
   a.part = s->lcPart.p;
@@ -1111,17 +1112,17 @@ static int lc_als(mlcp_session* s, LcArgs& a, int nitn, bool final_norm) {
   for (int it = 0; it < nitn; it++)
     for (int k = 0; k < D; k++) {
       if (s->Rmax <= 16)
-        PLAUNCH(KID_LC_RHS, k_lc_rhs<1>, dim3(nj), dim3(MLCP_NT), 0, a,
-                (const int2*)s->lcJobs.p, k);
+        PLAUNCH(KID_LC_RHS, k_lc_rhs<1>, dim3(B * a.ngrp), dim3(MLCP_NT), 0,
+                a, k);
       else if (s->Rmax <= 32)
-        PLAUNCH(KID_LC_RHS, k_lc_rhs<2>, dim3(nj), dim3(MLCP_NT), 0, a,
-                (const int2*)s->lcJobs.p, k);
+        PLAUNCH(KID_LC_RHS, k_lc_rhs<2>, dim3(B * a.ngrp), dim3(MLCP_NT), 0,
+                a, k);
       else if (s->Rmax <= 48)
-        PLAUNCH(KID_LC_RHS, k_lc_rhs<3>, dim3(nj), dim3(MLCP_NT), 0, a,
-                (const int2*)s->lcJobs.p, k);
+        PLAUNCH(KID_LC_RHS, k_lc_rhs<3>, dim3(B * a.ngrp), dim3(MLCP_NT), 0,
+                a, k);
       else
-        PLAUNCH(KID_LC_RHS, k_lc_rhs<4>, dim3(nj), dim3(MLCP_NT), 0, a,
-                (const int2*)s->lcJobs.p, k);
+        PLAUNCH(KID_LC_RHS, k_lc_rhs<4>, dim3(B * a.ngrp), dim3(MLCP_NT), 0,
+                a, k);
       size_t lds_s = ((size_t)s->Rmax * (s->nbas[k] | 1) +
                       (size_t)s->Rmax * s->Rmax) * sizeof(double);
       if (s->Rmax <= 16)
