@@ -79,8 +79,9 @@ struct OpArgs {
   const int* hs_se;      // [slot] one past its last sub-slot
   int nsubmax;           // most sub-slots of a mode (LDS rows of their sums)
   const double* hr_coef; // [record] term coefficient
-  const int4* hr_ops;    // [record] other operators (-1: none); a term
-                         // with more than four: x = term id, y = -2
+  const int2* hr_ops;    // [record] other operators, 16-bit indices:
+                         // x = o1 | o2 << 16, y = o3 | o4 << 16 (nops: none);
+                         // a term with more than four: x = term id
   double* M;             // [B][nops][rstr]  M(i,l) = <F_l, h Fold_i>
   double* G;             // [B][ndof][rstr]  G(i,l) = <F_l, F_i>
   double* Y;             // [B][nops][ystr]  Y(:,i) = h Fold_i

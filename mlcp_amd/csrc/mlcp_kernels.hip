@@ -551,23 +551,21 @@ __global__ __launch_bounds__(MLCP_NT) void k_op_refresh(OpArgs a,
 // wave's registers: lane j holds record base + j (operands and coefficient).
 struct RecWin {
   int base;
-  int ox, oy, oz, ow;
+  int oxy, ozw;   // operators x | y << 16, z | w << 16 (a term id in oxy)
   int clo, chi;
 };
 
-__device__ __forceinline__ void recwin_load(RecWin& w, const int4* ops,
+__device__ __forceinline__ void recwin_load(RecWin& w, const int2* ops,
                                             const double* coef, int base,
                                             int re, int lane) {
   // clamped to the last record of the range: every lane loads, always in
   // bounds; entries past the range are never read
   const int r = min(base + lane, re - 1);
-  const int4 o = ops[r];
+  const int2 o = ops[r];
   const double c = coef[r];
   w.base = base;
-  w.ox = o.x;
-  w.oy = o.y;
-  w.oz = o.z;
-  w.ow = o.w;
+  w.oxy = o.x;
+  w.ozw = o.y;
   w.clo = __double2loint(c);
   w.chi = __double2hiint(c);
 }
@@ -576,7 +574,7 @@ __device__ __forceinline__ void recwin_load(RecWin& w, const int4* ops,
 // the current window: when r leaves it, the prefetched window becomes the
 // current one and the window after it is requested.
 __device__ __forceinline__ void recwin_at(RecWin& cur, RecWin& nxt,
-                                          const int4* ops, const double* coef,
+                                          const int2* ops, const double* coef,
                                           int r, int re, int lane) {
   while (r - cur.base >= 64) {   // uniform
     cur = nxt;
@@ -626,10 +624,14 @@ __device__ __forceinline__ double had_run(const RecWin& w, int j,
   double t[U];
 #pragma unroll
   for (int q = 0; q < U; q++) {
-    double p = mc[__builtin_amdgcn_readlane(w.ox, j + q) * E];
-    if constexpr (NX >= 2) p *= mc[__builtin_amdgcn_readlane(w.oy, j + q) * E];
-    if constexpr (NX >= 3) p *= mc[__builtin_amdgcn_readlane(w.oz, j + q) * E];
-    if constexpr (NX >= 4) p *= mc[__builtin_amdgcn_readlane(w.ow, j + q) * E];
+    const int xy = __builtin_amdgcn_readlane(w.oxy, j + q);
+    double p = mc[(xy & 0xffff) * E];
+    if constexpr (NX >= 2) p *= mc[((unsigned)xy >> 16) * E];
+    if constexpr (NX >= 3) {
+      const int zw = __builtin_amdgcn_readlane(w.ozw, j + q);
+      p *= mc[(zw & 0xffff) * E];
+      if constexpr (NX >= 4) p *= mc[((unsigned)zw >> 16) * E];
+    }
     t[q] = p;
   }
 #pragma unroll
@@ -747,7 +749,7 @@ __global__ __launch_bounds__(HAD_NT) void k_op_hadamard(OpArgs a,
           for (int r = tb; r < te; r++) {
             recwin_at(cur, nxt, a.hr_ops, a.hr_coef, r, re, lane);
             const int j = r - cur.base;
-            const cptr<int> to = top + (long)__builtin_amdgcn_readlane(cur.ox, j) * D;
+            const cptr<int> to = top + (long)__builtin_amdgcn_readlane(cur.oxy, j) * D;
             double pr = recwin_coef(cur, j);
             for (int m = 1; m < D; m++) pr *= mc[to[(k + m) % D] * E];
             acc += pr;

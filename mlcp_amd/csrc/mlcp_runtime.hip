@@ -204,12 +204,12 @@ struct mlcp_session {
   int nsubmax = 1;
   std::vector<unsigned> hg_mask;
   std::vector<double> hr_coef;
-  std::vector<int4> hr_ops;
+  std::vector<int2> hr_ops;
   DBuf<int> d_hs_off, d_hs_beg, d_hs_end, d_hg_tb, d_hg_te;
   DBuf<int> d_hss_off, d_hss_gb, d_hss_ge, d_hs_sb, d_hs_se;
   DBuf<unsigned> d_hg_mask;
   DBuf<double> d_hr_coef;
-  DBuf<int4> d_hr_ops;
+  DBuf<int2> d_hr_ops;
   // per-state tables
   DBuf<int> d_colQ, d_colW, d_R, d_colstate;
   DBuf<double> d_pen;
@@ -288,6 +288,9 @@ static int build_terms(mlcp_session* s, const int* opid, const int* mptr,
                        int nops, int msize) {
   int D = s->ndof;
   if (T < 1 || nops < 1 || msize < 1) return fail("empty term list");
+  // the Hadamard records keep operator indices (and the index nops of the
+  // unit entry) in 16 bits
+  if (nops >= 65535) return fail("more than 65534 operators");
   s->T = T;
   s->nops = nops;
   s->opmode.assign(nops, -1);
@@ -428,7 +431,14 @@ static int build_terms(mlcp_session* s, const int* opid, const int* mptr,
               if (d != k && !((masks[g] >> d) & 1u)) o[nx++] = s->top[t * D + d];
           }
           s->hr_coef.push_back(hcoef[t]);
-          s->hr_ops.push_back(make_int4(o[0], o[1], o[2], o[3]));
+          // two 16-bit operator indices per word (a term id in the first
+          // word, whole, for the groups of mask bit 31)
+          if (masks[g] >> 31)
+            s->hr_ops.push_back(make_int2(o[0], 0));
+          else
+            s->hr_ops.push_back(
+                make_int2((int)((unsigned)o[0] | ((unsigned)o[1] << 16)),
+                          (int)((unsigned)o[2] | ((unsigned)o[3] << 16))));
         }
         s->hg_te.push_back((int)s->hr_coef.size());
       }
