@@ -1183,16 +1183,29 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
   constexpr int NT = NX / 4;   // 16-wide tiles of the mode-k basis
   extern __shared__ double sm[];
   __shared__ double red[8];
-  // Workgroups are dealt to the XCDs round robin; with a.xst = 8 only every
-  // 8th one works, which keeps all Gram-Schmidt traffic in one XCD's L2.
+  // Workgroups are dealt to the 8 XCDs round robin.  With a.xst = 8 only
+  // every 8th one works, which keeps all Gram-Schmidt traffic in one XCD's
+  // L2; with a.xst = 1 workgroup b runs job (l, ic) with l = b % 8 + 8 q,
+  // q = (b / 8) / nch, ic = (b / 8) % nch: the column groups of previous
+  // state l all run on XCD l % 8 and read its factors through one L2 (the
+  // grid is 8 ceil((iv + 1) / 8) nch; l > iv exits).
   if (blockIdx.x % a.xst) return;
   if (a.done && a.done[a.iv]) return;   // converged state: step skipped
-  const int job = blockIdx.x / a.xst;
   const int iv = a.iv, D = a.p.ndof;
   const int R = a.Riv, cF = a.cFiv, n = a.p.nbas[k];
 
   const int nch = (R + GS_ICH - 1) / GS_ICH;
-  const int l = job / nch, ic = job % nch;
+  int l, ic;
+  if (a.xst == 1) {
+    const int q = blockIdx.x >> 3, lq = q / nch;
+    ic = q - lq * nch;
+    l = (blockIdx.x & 7) + 8 * lq;
+    if (l > iv) return;
+  } else {
+    const int job = blockIdx.x / a.xst;
+    l = job / nch;
+    ic = job % nch;
+  }
   const int i0 = ic * GS_ICH, ni = min(GS_ICH, R - i0);
   const bool self = (l == iv);
   const int Rl = self ? R : a.R[l], cl = self ? cF : a.col[l];

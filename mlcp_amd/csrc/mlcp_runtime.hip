@@ -1005,8 +1005,8 @@ static int launch_ortho(mlcp_session* s, int nals) {
     int nch = (s->ranks[iv] + GS_ICH - 1) / GS_ICH;
     for (int it = 0; it < nals; it++)
       for (int k = 0; k < D; k++) {
-        LAUNCH_NX(KID_GS_TERMS, k_gs_terms, dim3((iv + 1) * nch * g.xst),
-                  lds_t, g, k, kprev);
+        const int nterm = g.xst == 1 ? 8 * ((iv + 8) / 8) * nch : (iv + 1) * nch * g.xst;
+        LAUNCH_NX(KID_GS_TERMS, k_gs_terms, dim3(nterm), lds_t, g, k, kprev);
         // workgroup 0 of k_gs_reduce factors the system matrix, the others
         // sum the right-hand side; the last one to finish solves
         const int nred = 1 + (s->ranks[iv] * s->nbas[k] + MLCP_NT - 1) / MLCP_NT;
