@@ -202,7 +202,10 @@ def test_update_phase_ill_conditioned(hip):
         assert e < 1e-7, (i, e)
 
 
-@pytest.mark.parametrize("n", [1, 2, 7, 33, 64, 130, 200])
+# 3: the smallest order with a Householder step; 131 / 132: the largest
+# order whose working matrix is held in LDS and the smallest in global
+# memory; 256: the largest block
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 33, 64, 130, 131, 132, 200, 256])
 def test_gep_solver(hip, n):
     from mlcp_amd.intertwine import gpu_gep
     from oracle.orc import C
@@ -219,6 +222,36 @@ def test_gep_solver(hip, n):
     assert np.max(np.abs(A @ X - B @ X * w)) < 1e-9 * max(1, np.max(np.abs(w)))
     assert np.max(np.abs(X.T @ B @ X - np.eye(n))) < 1e-10
     assert np.max(np.abs(X - Xo)) < 1e-7
+
+
+@pytest.mark.parametrize("kind", ["triple", "diagonal", "identity"])
+def test_gep_solver_degenerate(hip, kind):
+    """Repeated eigenvalues (each three times, in a random basis), an
+    already diagonal matrix (every Householder reflection skipped) and A = B
+    (all eigenvalues 1): the eigenvectors are not unique, so the check is
+    the eigenvalues against the oracle, the residual and B-orthonormality."""
+    from mlcp_amd.intertwine import gpu_gep
+    from oracle.orc import C
+    say("gep degenerate", kind)
+    n = 48
+    rng = np.random.default_rng(11)
+    b = rng.normal(size=(n, n)) * 0.1
+    B = np.eye(n) + b @ b.T
+    if kind == "triple":
+        q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        A = q @ np.diag(np.repeat(np.arange(n // 3) * 7.0 - 20.0, 3)) @ q.T
+        A = 0.5 * (A + A.T)
+    elif kind == "diagonal":
+        A = np.diag(rng.normal(size=n) * 10.0)
+        B = np.eye(n)
+    else:
+        A = B.copy()
+    w, X = gpu_gep(A, B)
+    wo, _ = C.dsygv(A, B)
+    scale = max(1, np.max(np.abs(wo)))
+    assert np.max(np.abs(w - wo)) < 1e-10 * scale
+    assert np.max(np.abs(A @ X - B @ X * w)) < 1e-9 * scale
+    assert np.max(np.abs(X.T @ B @ X - np.eye(n))) < 1e-10
 
 
 @pytest.mark.parametrize("name,nb,ranks,ncoup", CASES[:2])
