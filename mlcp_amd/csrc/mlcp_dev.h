@@ -145,6 +145,9 @@ struct GsArgs {
   int pstr;
   int nmax;
   double* X;             // [B][ndof][rstr]  X_l(j,i) = <Q_l,j, F_i>
+  double* XS;            // [B][ndof][rstr]  per previous state: suffix
+                         // products prod_{d>k} X_l^d (slot k < ndof-1) and
+                         // the sweep's prefix product (slot ndof-1)
   double* G;             // [ndof][rstr]     own Gram
   double* part;          // [iv+1][pstr]  P_l(i,m) at i*n + m
   double* pc;            // [iv][GS_PCS]  partial Gram-Schmidt coefficients

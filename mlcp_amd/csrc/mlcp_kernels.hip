@@ -1177,6 +1177,34 @@ __global__ __launch_bounds__(MLCP_NT) void k_normcoef(Pool p, int c0, int R) {
 // never leaves the registers.  Each wave forms the part of P summed over its
 // own rows; the four parts are added in LDS (4 * NX * 256 doubles of dynamic
 // LDS, n <= 4*NX).  All global operands are loaded before the first use.
+// X^d(j, i) = sum_m Q_l,j^d(m) F_i^d(m) for this lane's entries of the
+// cross Gram of state l (first column cl, rank Rl) with the 16 columns
+// i0.. of state iv (first column cF): rows 16g.., MFMA result layout
+// (element v: j = 16g + lq + 4v, i = i0 + lr); operands past Rl, the
+// columns and the basis are zero.
+__device__ __forceinline__ v4d gs_cross(const GsArgs& a, int d, int cl, int Rl,
+                                        int cF, int i0, bool icol, int g,
+                                        int lr, int lq) {
+  const int nd = a.p.nbas[d], nks = (nd + 3) >> 2;
+  const int ja = 16 * g + lr;
+  const double* qa = colp(a.p, d, cl + (ja < Rl ? ja : 0));
+  const double* fb = colp(a.p, d, cF + i0 + (icol ? lr : 0));
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
+  for (int ks0 = 0; ks0 < nks; ks0 += 8) {
+    double av[8], bv[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const int m = 4 * (ks0 + b) + lq;
+      av[b] = (ja < Rl && m < nd) ? qa[m] : 0.0;
+      bv[b] = (icol && m < nd) ? fb[m] : 0.0;
+    }
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+      acc = mfma_f64_4(av[b], bv[b], acc);   // zero operands past nd
+  }
+  return acc;
+}
+
 template <int NX>
 __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
                                                        int kprev) {
@@ -1244,54 +1272,103 @@ __global__ __launch_bounds__(MLCP_NT) void k_gs_terms(GsArgs a, int k,
         qk[v][t] = (jr && m < n) ? q[m] : 0.0;
       }
     }
-    // cached cross Grams of the modes not refreshed here, four modes a round
-    for (int d0 = 0; d0 < D; d0 += 4) {
-      double xv[4][4];
+    if (self) {
+      // own Grams G_d (written by the previous step's solve), every mode
+      // at the state's first step from the factors
+      for (int d0 = 0; d0 < D; d0 += 4) {
+        double xv[4][4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int d = min(d0 + u, D - 1);
+        for (int u = 0; u < 4; u++) {
+          const int d = min(d0 + u, D - 1);
 #pragma unroll
-        for (int v = 0; v < 4; v++) xv[u][v] = Xl[(long)d * a.rstr + xo[v]];
-      }
+          for (int v = 0; v < 4; v++) xv[u][v] = Xl[(long)d * a.rstr + xo[v]];
+        }
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int d = d0 + u;
-        const bool fresh = self ? (kprev < 0) : (kprev < 0 || d == kprev);
-        if (d < D && !fresh) {
+        for (int u = 0; u < 4; u++) {
+          const int d = d0 + u;
+          if (d < D && kprev >= 0) {
 #pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const double x = ok[v] ? xv[u][v] : 0.0;
-            if (d == k) xk[v] = x; else ps[v] *= x;
+            for (int v = 0; v < 4; v++) {
+              const double x = ok[v] ? xv[u][v] : 0.0;
+              if (d == k) xk[v] = x; else ps[v] *= x;
+            }
           }
         }
       }
-    }
-    // refreshed cross Grams: X^d(j, i) = sum_m Q_l,j^d(m) F_i^d(m)
-    const int d0 = (self || kprev < 0) ? 0 : kprev;
-    const int d1 = (kprev < 0) ? D : (self ? 0 : kprev + 1);
-    for (int d = d0; d < d1; d++) {
-      const int nd = a.p.nbas[d], nks = (nd + 3) >> 2;
-      const int ja = 16 * g + lr;
-      const double* qa = colp(a.p, d, cl + (ja < Rl ? ja : 0));
-      const double* fb = colp(a.p, d, cF + i0 + (icol ? lr : 0));
-      v4d acc = {0.0, 0.0, 0.0, 0.0};
-      for (int ks0 = 0; ks0 < nks; ks0 += 8) {
-        double av[8], bv[8];
+      if (kprev < 0)
+        for (int d = 0; d < D; d++) {
+          const v4d x = gs_cross(a, d, cl, Rl, cF, i0, icol, g, lr, lq);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-          const int m = 4 * (ks0 + b) + lq;
-          av[b] = (ja < Rl && m < nd) ? qa[m] : 0.0;
-          bv[b] = (icol && m < nd) ? fb[m] : 0.0;
+          for (int v = 0; v < 4; v++) {
+            const double xx = ok[v] ? x[v] : 0.0;
+            if (ok[v]) Xl[(long)d * a.rstr + xo[v]] = xx;
+            if (d == k) xk[v] = xx; else ps[v] *= xx;
+          }
+        }
+    } else {
+      // Cross Grams of a previous state.  PS = prod_{d != k} X^d is formed
+      // from two cached products: the prefix A = prod_{d < k} X^d (slot
+      // D-1 of XS, extended by the refreshed mode k-1 at every step) and
+      // the suffix S_k = prod_{d > k} X^d (slot k of XS, formed once per
+      // sweep at its first step, when every mode above 0 is current), so
+      // a step reads three arrays instead of D-1.
+      double* XSl = a.XS + (long)l * D * a.rstr;
+      if (k == 0) {
+        // sweep start: refresh mode D-1 (every mode at the state's first
+        // step), then the suffixes S_{d-1} = X^d S_d from the top
+        for (int d = (kprev < 0 ? 0 : D - 1); d < D; d++) {
+          const v4d x = gs_cross(a, d, cl, Rl, cF, i0, icol, g, lr, lq);
+#pragma unroll
+          for (int v = 0; v < 4; v++)
+            if (ok[v]) Xl[(long)d * a.rstr + xo[v]] = x[v];
+        }
+        double sv[4] = {1.0, 1.0, 1.0, 1.0};
+        for (int d1 = D - 1; d1 >= 1; d1 -= 4) {
+          double xv[4][4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int d = max(d1 - u, 0);
+#pragma unroll
+            for (int v = 0; v < 4; v++) xv[u][v] = Xl[(long)d * a.rstr + xo[v]];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int d = d1 - u;
+            if (d >= 1) {
+#pragma unroll
+              for (int v = 0; v < 4; v++) {
+                sv[v] = d == D - 1 ? xv[u][v] : xv[u][v] * sv[v];
+                if (ok[v]) XSl[(long)(d - 1) * a.rstr + xo[v]] = sv[v];
+              }
+            }
+          }
         }
 #pragma unroll
-        for (int b = 0; b < 8; b++)
-          acc = mfma_f64_4(av[b], bv[b], acc);   // zero operands past nd
-      }
+        for (int v = 0; v < 4; v++) {
+          ps[v] = ok[v] ? sv[v] : 0.0;
+          xk[v] = ok[v] ? Xl[xo[v]] : 0.0;
+        }
+      } else {
+        // step k >= 1: the prefix before mode k-1, the suffix above k and
+        // X^k requested first, then mode k-1 refreshed
+        double ap[4], sk[4], xkv[4];
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const double x = ok[v] ? acc[v] : 0.0;
-        if (ok[v]) Xl[(long)d * a.rstr + xo[v]] = x;
-        if (d == k) xk[v] = x; else ps[v] *= x;
+        for (int v = 0; v < 4; v++) {
+          ap[v] = XSl[(long)(D - 1) * a.rstr + xo[v]];
+          sk[v] = XSl[(long)min(k, D - 2) * a.rstr + xo[v]];
+          xkv[v] = Xl[(long)k * a.rstr + xo[v]];
+        }
+        const v4d x = gs_cross(a, k - 1, cl, Rl, cF, i0, icol, g, lr, lq);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const double am = k == 1 ? x[v] : ap[v] * x[v];
+          if (ok[v]) {
+            Xl[(long)(k - 1) * a.rstr + xo[v]] = x[v];
+            if (k < D - 1) XSl[(long)(D - 1) * a.rstr + xo[v]] = am;
+          }
+          ps[v] = ok[v] ? (k < D - 1 ? am * sk[v] : am) : 0.0;
+          xk[v] = ok[v] ? xkv[v] : 0.0;
+        }
       }
     }
   }

@@ -220,7 +220,7 @@ struct mlcp_session {
   std::vector<int> njobs_mode;
   int njobs_all = 0, njobs_had = 0;
   // Gram-Schmidt
-  DBuf<double> gsX, gsG, gsPart, gsBbs, gsPc, gsBjk, gsGjf;
+  DBuf<double> gsX, gsXS, gsG, gsPart, gsBbs, gsPc, gsBjk, gsGjf;
   DBuf<int> gsCnt;
   // form of the cycle (mlcp_hip_session_set_form): bit 1 = ALS phases to a
   // residual tolerance instead of the fixed sweep counts
@@ -611,7 +611,8 @@ static int session_alloc(mlcp_session* s) {
   if (build_had_jobs(s)) return -1;
   // Gram-Schmidt
   int pstr = s->Rmax * s->nmax;
-  if (s->gsX.need((size_t)B * D * rstr) || s->gsG.need((size_t)D * rstr) ||
+  if (s->gsX.need((size_t)B * D * rstr) || s->gsXS.need((size_t)B * D * rstr) ||
+      s->gsG.need((size_t)D * rstr) ||
       s->gsPart.need((size_t)(B + 1) * pstr) || s->gsBbs.need(rstr) ||
       s->gsPc.need((size_t)(B + 1) * GS_PCS) || s->gsBjk.need(pstr) ||
       s->gsGjf.need(GJF_SIZE) || s->gsCnt.need(1) || s->gsDone.need(B) ||
@@ -936,6 +937,7 @@ static GsArgs gs_args(mlcp_session* s, int iv) {
   g.pstr = s->Rmax * s->nmax;
   g.nmax = s->nmax;
   g.X = s->gsX.p;
+  g.XS = s->gsXS.p;
   g.G = s->gsG.p;
   g.part = s->gsPart.p;
   g.bbs = s->gsBbs.p;
@@ -2367,8 +2369,8 @@ extern "C" void intwdevicemem_(const int* rk, const int* ndof,
   const double opals = 8 * (B * O * r2 + B * D * r2 + B * O * nmax * R +
                             B * O * r2 + B * r2) +
                        8 * (2 * B * (O + D) + B * ((r2 + 63) / 64) * D * 4);
-  // Gram-Schmidt caches
-  const double gs = 8 * (B * D * r2 + D * r2 + (B + 1) * R * nmax + r2 +
+  // Gram-Schmidt caches (cross Grams and their prefix / suffix products)
+  const double gs = 8 * (2 * B * D * r2 + D * r2 + (B + 1) * R * nmax + r2 +
                          (B + 1) * GS_PCS + R * nmax + GJF_SIZE) + 4;
   // pair jobs, QHQ and S, U, eigenvalues, eigensolver work
   const double gep = 24.0 * B * (B + 1) + 8 * (2 * B * B + B * B + B +
