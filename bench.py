@@ -291,7 +291,7 @@ def main():
         print(json.dumps(pmc_summary(args.pmc_summary), indent=1))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must "
+        print(json.dumps({"error": "mlcp_amd runs on one GPU; --gpus must "
                           "be 1 (multi-GPU sharding is a design note only)"}))
         sys.exit(2)
 

@@ -6,7 +6,7 @@ counts), on the GPU through the C-ABI:
 * mixed form: the power sweeps apply the terms on the FP32 matrix cores;
 * learned warm start of the reduced H*Q fit (trained on plain cycles).
 
-Each is judged the way the north star states it: one cycle from the
+Each is judged by the benchmark's accuracy rule: one cycle from the
 teacher-forced fixture block (tests/golden/teacher_<name>.npz) against the
 FP64 oracle's cycle from the same block, every eigenvalue within
 0.01 cm^-1.  Measured against the oracle: tolerance form at Gram-Schmidt

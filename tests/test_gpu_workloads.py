@@ -20,7 +20,7 @@ Two kinds of check (tests/golden/):
   late block of the solve (after nine cycles), the same block and shift the
   oracle ran.  This isolates the implementation from the trajectory's
   amplification: every eigenvalue must agree within 0.01 cm^-1 (the
-  north-star tolerance), and is in fact held to 1e-6 cm^-1.
+  accuracy target of the benchmark), and is in fact held to 1e-6 cm^-1.
 
 Tolerances (cm^-1, or H units for hh111; |w_gpu - w_oracle| over all
 states), trajectory cycles 1, 2, 3, ...:
@@ -43,7 +43,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 BOUNDS = {"hh111": [1e-3] * 10, "qff12": [1e-7, 1e-4, 1e-2],
           "qff24": [1e-7, 1e-4, 1e-2]}
-TF_TOL = 0.01        # the north-star eigenvalue tolerance
+TF_TOL = 0.01        # the benchmark's eigenvalue accuracy target (cm^-1)
 TF_REG = 1e-6        # what the implementation holds (regression bound)
 
 
